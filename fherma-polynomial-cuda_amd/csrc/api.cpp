@@ -1,0 +1,363 @@
+// C ABI of the gfx950 negacyclic multiplier (include/polymul.h): plan
+// lifetime, device memory, the per-product stage sequence and parity taps.
+// The stage sequence follows fherma_run's non-pipelined path
+// (candidates/ordered-streaming/solve.cu:1230-1270): H2D, conversion, NTT
+// stages, CRT GEMM, reconstruction, D2H, with integer-only reconstruction for
+// every q instead of the reference's FP64 fallbacks (solve.cu:718-895).
+#include "polymul.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoDevice : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* where) {
+    if (e != hipSuccess) throw DeviceError(std::string(where) + ": " + hipGetErrorString(e));
+}
+
+constexpr int kStages = 7;
+
+}  // namespace
+
+struct pm_plan {
+    pm::HostPlan host;
+    bool device = false;
+    int device_id = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    pm::DeviceTables tables;
+    pm::DeviceWork work;
+    uint32_t* io_a = nullptr;  // device copies for the host-vector entry point
+    uint32_t* io_b = nullptr;
+    uint32_t* io_c = nullptr;
+    uint64_t device_bytes = 0;
+    bool profiling = false;
+    hipEvent_t events[kStages] = {};
+    float stage_ms[kStages] = {};
+
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        size_t bytes = count * sizeof(T);
+        if (bytes == 0) bytes = 256;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+            hip_check(e, "hipMalloc");
+        }
+        allocations.push_back(p);
+        device_bytes += bytes;
+        return static_cast<T*>(p);
+    }
+
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload plan");
+        return p;
+    }
+
+    ~pm_plan() {
+        if (device) {
+            hipSetDevice(device_id);
+            if (stream) hipStreamSynchronize(stream);
+            for (hipEvent_t& e : events)
+                if (e) hipEventDestroy(e);
+            for (void* p : allocations) hipFree(p);
+            if (stream) hipStreamDestroy(stream);
+        }
+    }
+
+    void make_device() {
+        hip_check(hipGetDevice(&device_id), "hipGetDevice");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        device = true;
+        for (hipEvent_t& e : events) hip_check(hipEventCreate(&e), "hipEventCreate");
+        const pm::Shape& s = host.shape;
+        const size_t nl = size_t(s.n) * s.limbs;
+        io_a = alloc<uint32_t>(nl);
+        io_b = alloc<uint32_t>(nl);
+        io_c = alloc<uint32_t>(nl);
+        if (s.trivial) return;
+        tables.consts = upload(host.consts);
+        tables.wconv = upload(host.wconv);
+        tables.tw_fwd = reinterpret_cast<const uint2*>(upload(host.tw_fwd));
+        tables.tw_inv = reinterpret_cast<const uint2*>(upload(host.tw_inv));
+        tables.wcrt = upload(host.wcrt);
+        tables.q = upload(host.q);
+        tables.neg_p = upload(host.neg_p_mod_q);
+        tables.corr_t = upload(host.corr_t);
+        tables.corr_e = upload(host.corr_e);
+        // planes: [2][r][N]; after the row kernel the second plane is free and
+        // holds the reconstruction's T scratch ([L+2][N]).
+        size_t plane = size_t(s.r) * s.n;
+        size_t scratch = size_t(s.t_words) * s.n;
+        work.planes = alloc<uint32_t>(plane + (plane > scratch ? plane : scratch));
+        work.t_words = work.planes + plane;
+        work.words = alloc<int64_t>(size_t(s.crt_words) * s.n);
+    }
+
+    // Runs stages up to and including `last` (PM_STAGE_*) on device operands.
+    void run(const uint32_t* a, const uint32_t* b, uint32_t* c, int last, hipStream_t st) {
+        const pm::Shape& s = host.shape;
+        if (s.trivial) {
+            pm::launch_zero(c, size_t(s.n) * s.limbs, st);
+            return;
+        }
+        auto mark = [&](int i) {
+            if (profiling) hip_check(hipEventRecord(events[i], st), "hipEventRecord");
+        };
+        mark(0);
+        pm::launch_conversion(s, tables, a, b, work.planes, st);
+        mark(1);
+        if (last == PM_STAGE_RESIDUES) return;
+        pm::launch_ntt_forward_columns(s, tables, work.planes, st);
+        mark(2);
+        pm::launch_ntt_rows_product(s, tables, work.planes, st);
+        mark(3);
+        pm::launch_ntt_inverse_columns(s, tables, work.planes, st);
+        mark(4);
+        if (last == PM_STAGE_PRODUCT) return;
+        pm::launch_crt(s, tables, work.planes, work.words, st);
+        mark(5);
+        if (last == PM_STAGE_WORDS) return;
+        pm::launch_reconstruct(s, tables, work.words, work.t_words, c, st);
+        mark(6);
+    }
+
+    void collect_times() {
+        if (!profiling || host.shape.trivial) return;
+        for (int i = 0; i < kStages - 1; ++i)
+            hip_check(hipEventElapsedTime(&stage_ms[i], events[i], events[i + 1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&stage_ms[kStages - 1], events[0], events[kStages - 1]), "hipEventElapsedTime");
+    }
+};
+
+namespace {
+
+template <class F>
+pm_status guarded(F&& body) {
+    try {
+        body();
+        return PM_OK;
+    } catch (const NoDevice& e) {
+        g_last_error = e.what();
+        return PM_ERR_NO_DEVICE;
+    } catch (const DeviceError& e) {
+        g_last_error = e.what();
+        return PM_ERR_DEVICE;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return PM_ERR_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of memory";
+        return PM_ERR_OUT_OF_MEMORY;
+    } catch (const std::runtime_error& e) {
+        g_last_error = e.what();
+        return std::string(e.what()).find("insufficient independent NTT fields") != std::string::npos
+                   ? PM_ERR_INSUFFICIENT_FIELDS
+                   : PM_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PM_ERR_INTERNAL;
+    }
+}
+
+void require_device(const pm_plan* plan) {
+    if (!plan) throw std::invalid_argument("null plan");
+    if (!plan->device) throw NoDevice("plan has no device state (created with pm_plan_create_host)");
+}
+
+pm_status create(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, pm_plan** out, bool device) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null output pointer");
+        *out = nullptr;
+        auto plan = std::make_unique<pm_plan>();
+        plan->host = pm::build_plan(n, width, limbs, q);
+        if (device) plan->make_device();
+        *out = plan.release();
+    });
+}
+
+template <class T>
+void copy_table(const std::vector<T>& v, void* out, size_t* bytes) {
+    size_t need = v.size() * sizeof(T);
+    if (out) {
+        if (*bytes < need) throw std::invalid_argument("table buffer too small");
+        std::memcpy(out, v.data(), need);
+    }
+    *bytes = need;
+}
+
+}  // namespace
+
+extern "C" {
+
+pm_status pm_plan_create(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, pm_plan** out) {
+    return create(n, width, limbs, q, out, true);
+}
+
+pm_status pm_plan_create_host(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, pm_plan** out) {
+    return create(n, width, limbs, q, out, false);
+}
+
+void pm_plan_destroy(pm_plan* plan) { delete plan; }
+
+const char* pm_last_error(void) { return g_last_error.c_str(); }
+
+pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    return guarded([&] {
+        require_device(plan);
+        if (!a || !b || !c) throw std::invalid_argument("null operand");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        const pm::Shape& s = plan->host.shape;
+        const size_t bytes = size_t(s.n) * s.limbs * sizeof(uint32_t);
+        hipStream_t st = plan->stream;
+        hip_check(hipMemcpyAsync(plan->io_a, a, bytes, hipMemcpyHostToDevice, st), "copy a to device");
+        hip_check(hipMemcpyAsync(plan->io_b, b, bytes, hipMemcpyHostToDevice, st), "copy b to device");
+        plan->run(plan->io_a, plan->io_b, plan->io_c, PM_STAGE_OUTPUT, st);
+        hip_check(hipMemcpyAsync(c, plan->io_c, bytes, hipMemcpyDeviceToHost, st), "copy c to host");
+        hip_check(hipStreamSynchronize(st), "multiply");
+        plan->collect_times();
+    });
+}
+
+pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream) {
+    return guarded([&] {
+        require_device(plan);
+        if (!a || !b || !c) throw std::invalid_argument("null operand");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->stream;
+        plan->run(a, b, c, PM_STAGE_OUTPUT, st);
+        hip_check(hipStreamSynchronize(st), "multiply");
+        plan->collect_times();
+    });
+}
+
+pm_status pm_plan_get_info(const pm_plan* plan, pm_plan_info* info) {
+    return guarded([&] {
+        if (!plan || !info) throw std::invalid_argument("null argument");
+        const pm::Shape& s = plan->host.shape;
+        info->n = s.n;
+        info->width = s.width;
+        info->limbs = s.limbs;
+        info->primes = s.r;
+        info->block = s.n1;
+        info->transform = s.n2;
+        info->row_width = s.row_width;
+        info->reference_walk = s.reference_walk ? 1 : 0;
+        info->qbits = s.qbits;
+        info->trivial = s.trivial ? 1 : 0;
+        info->device_bytes = plan->device_bytes;
+    });
+}
+
+pm_status pm_plan_get_primes(const pm_plan* plan, uint32_t* primes, size_t capacity) {
+    return guarded([&] {
+        if (!plan || (!primes && capacity)) throw std::invalid_argument("null argument");
+        const auto& v = plan->host.primes;
+        for (size_t i = 0; i < v.size() && i < capacity; ++i) primes[i] = v[i];
+    });
+}
+
+pm_status pm_plan_export_table(const pm_plan* plan, int table, void* out, size_t* bytes) {
+    return guarded([&] {
+        if (!plan || !bytes) throw std::invalid_argument("null argument");
+        const pm::HostPlan& h = plan->host;
+        switch (table) {
+            case PM_TABLE_PRIME_CONSTS: copy_table(h.consts, out, bytes); break;
+            case PM_TABLE_CONV_WEIGHTS: copy_table(h.wconv, out, bytes); break;
+            case PM_TABLE_TWIDDLES_FWD: copy_table(h.tw_fwd, out, bytes); break;
+            case PM_TABLE_TWIDDLES_INV: copy_table(h.tw_inv, out, bytes); break;
+            case PM_TABLE_CRT_WEIGHTS: copy_table(h.wcrt, out, bytes); break;
+            case PM_TABLE_M_MOD_Q: copy_table(h.m_mod_q, out, bytes); break;
+            case PM_TABLE_NEG_P_MOD_Q: copy_table(h.neg_p_mod_q, out, bytes); break;
+            case PM_TABLE_CORR_T: copy_table(h.corr_t, out, bytes); break;
+            case PM_TABLE_CORR_E: copy_table(h.corr_e, out, bytes); break;
+            case PM_TABLE_RECIPROCALS: copy_table(h.reciprocal, out, bytes); break;
+            case PM_TABLE_BETA: copy_table(h.beta, out, bytes); break;
+            case PM_TABLE_SHAPE: {
+                const pm::Shape& s = h.shape;
+                std::vector<uint32_t> w = {s.n,         s.width,     s.limbs,         s.n1,
+                                           s.n2,        s.log_n2,    s.r,             s.r_pad,
+                                           s.conv_k,    s.conv_rows, s.crt_words,     s.crt_words_pad,
+                                           s.crt_rows,  s.crt_k,     s.t_words,       s.row_width,
+                                           s.rows,      s.qbits,     s.s_shift,       s.k_shift,
+                                           uint32_t(s.mu), uint32_t(s.mu >> 32), s.trivial ? 1u : 0u,
+                                           s.reference_walk ? 1u : 0u};
+                copy_table(w, out, bytes);
+                break;
+            }
+            default: throw std::invalid_argument("unknown table");
+        }
+    });
+}
+
+pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, int stage, void* out, size_t* bytes) {
+    return guarded([&] {
+        if (!plan || !bytes) throw std::invalid_argument("null argument");
+        const pm::Shape& s = plan->host.shape;
+        size_t need = 0;
+        switch (stage) {
+            case PM_STAGE_RESIDUES: need = size_t(2) * s.r * s.n * 4; break;
+            case PM_STAGE_PRODUCT: need = size_t(s.r) * s.n * 4; break;
+            case PM_STAGE_WORDS: need = size_t(s.crt_words) * s.n * 8; break;
+            case PM_STAGE_OUTPUT: need = size_t(s.n) * s.limbs * 4; break;
+            default: throw std::invalid_argument("unknown stage");
+        }
+        if (!out) {
+            *bytes = need;
+            return;
+        }
+        if (*bytes < need) throw std::invalid_argument("stage buffer too small");
+        require_device(plan);
+        if (!a || !b) throw std::invalid_argument("null operand");
+        if (s.trivial && stage != PM_STAGE_OUTPUT) throw std::invalid_argument("q == 1 has no intermediate stages");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        const size_t nl = size_t(s.n) * s.limbs * 4;
+        hipStream_t st = plan->stream;
+        hip_check(hipMemcpyAsync(plan->io_a, a, nl, hipMemcpyHostToDevice, st), "copy a");
+        hip_check(hipMemcpyAsync(plan->io_b, b, nl, hipMemcpyHostToDevice, st), "copy b");
+        plan->run(plan->io_a, plan->io_b, plan->io_c, stage, st);
+        const void* src = stage == PM_STAGE_WORDS ? static_cast<const void*>(plan->work.words)
+                          : stage == PM_STAGE_OUTPUT ? static_cast<const void*>(plan->io_c)
+                                                     : static_cast<const void*>(plan->work.planes);
+        hip_check(hipMemcpyAsync(out, src, need, hipMemcpyDeviceToHost, st), "copy stage");
+        hip_check(hipStreamSynchronize(st), "debug stage");
+        *bytes = need;
+    });
+}
+
+pm_status pm_set_profiling(pm_plan* plan, int enabled) {
+    return guarded([&] {
+        if (!plan) throw std::invalid_argument("null plan");
+        plan->profiling = enabled != 0;
+    });
+}
+
+pm_status pm_get_stage_times(const pm_plan* plan, float* ms, int count) {
+    return guarded([&] {
+        if (!plan || (!ms && count > 0)) throw std::invalid_argument("null argument");
+        for (int i = 0; i < count && i < kStages; ++i) ms[i] = plan->stage_ms[i];
+    });
+}
+
+}  // extern "C"

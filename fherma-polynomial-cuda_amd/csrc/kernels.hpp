@@ -1,0 +1,44 @@
+// Launch wrappers for the gfx950 kernels of the negacyclic multiplier.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace pm {
+
+// Device views of the plan's tables.
+struct DeviceTables {
+    const PrimeConst* consts = nullptr;   // [r]
+    const int8_t* wconv = nullptr;        // [2][conv_rows][conv_k]
+    const uint2* tw_fwd = nullptr;        // [r][n2]
+    const uint2* tw_inv = nullptr;        // [r][n2]
+    const int8_t* wcrt = nullptr;         // [crt_rows][crt_k]
+    const uint32_t* q = nullptr;          // [L]
+    const uint32_t* neg_p = nullptr;      // [L]
+    const uint32_t* corr_t = nullptr;     // [t_words]
+    const uint32_t* corr_e = nullptr;     // [4]
+};
+
+// Working buffers.
+struct DeviceWork {
+    uint32_t* planes = nullptr;  // [2][r][N] residues; plane 0 receives v_k
+    int64_t* words = nullptr;    // [crt_words][N] CRT word sums
+    uint32_t* t_words = nullptr; // [t_words][N] T scratch (aliases planes)
+};
+
+// Stage launches, all on `stream`.
+void launch_conversion(const Shape& s, const DeviceTables& t, const uint32_t* a, const uint32_t* b,
+                       uint32_t* planes, hipStream_t stream);
+void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
+void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
+void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
+void launch_crt(const Shape& s, const DeviceTables& t, const uint32_t* product_plane, int64_t* words,
+                hipStream_t stream);
+void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
+                        uint32_t* c, hipStream_t stream);
+void launch_zero(uint32_t* c, size_t words, hipStream_t stream);
+
+}  // namespace pm

@@ -1,0 +1,77 @@
+// Parameter-only plan for c = a*b mod (X^N + 1, q): everything fherma_init
+// computes before any operand is seen (candidates/ordered-streaming/solve.cu:900-1040),
+// re-designed for the gfx950 kernels in kernels.hip.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bigint.hpp"
+
+namespace pm {
+
+// Per-prime constants read by the kernels.
+struct PrimeConst {
+    uint32_t p;      // NTT prime, 2^30 <= p < 2^31
+    uint32_t pinv;   // p^-1 mod 2^32 (subtractive Montgomery)
+    uint32_t off_a;  // conversion offset for operand a (signed-byte correction), mod p
+    uint32_t off_b;  // same for operand b
+};
+
+struct Shape {
+    uint32_t n = 0, width = 0, limbs = 0;
+    // Transform split: N = n1 * n2.  The NTT of length n2 runs over the n1
+    // interleaved sub-sequences a_{t + n1*u}; the base case multiplies degree<n1
+    // polynomials mod (X^n1 - zeta).  n1 = 1 is the plain negacyclic NTT.
+    uint32_t n1 = 1, n2 = 1, log_n2 = 0;
+    uint32_t r = 0;          // primes
+    uint32_t r_pad = 0;      // primes padded to a multiple of 32
+    uint32_t conv_k = 0;     // conversion GEMM K: 4L bytes padded to 64
+    uint32_t conv_rows = 0;  // conversion GEMM rows: 4 digits per (padded) prime
+    uint32_t crt_words = 0;  // CRT GEMM output words: L+1 sum words + 3 quotient words
+    uint32_t crt_words_pad = 0;
+    uint32_t crt_rows = 0;   // 4 bytes per (padded) word
+    uint32_t crt_k = 0;      // 4 bytes per (padded) prime
+    uint32_t t_words = 0;    // L+2: width of T = sum v_k M_k + K R
+    uint32_t row_width = 1;  // NTT row length C (flat elements); rows = N / C
+    uint32_t rows = 1;
+    uint32_t qbits = 0;      // bit length of q
+    uint32_t s_shift = 0;    // Barrett: Ts = floor(T / 2^s_shift)
+    uint32_t k_shift = 0;    // Barrett: Q = floor(Ts * mu / 2^k_shift)
+    uint64_t mu = 0;         // floor(2^(qbits-1+63) / q)
+    bool trivial = false;    // q == 1: every output is zero
+    bool reference_walk = true;  // primes are exactly fherma_init's walk (n1 == 1)
+};
+
+struct HostPlan {
+    Shape shape;
+    std::vector<uint32_t> primes;
+    std::vector<PrimeConst> consts;
+    std::vector<int8_t> wconv;        // [2][conv_rows][conv_k]
+    std::vector<uint32_t> tw_fwd;     // [r][n2][2]  (psi^bitrev(j), Shoup quotient)
+    std::vector<uint32_t> tw_inv;     // [r][n2][2]  (psi^-bitrev(j), Shoup quotient)
+    std::vector<int8_t> wcrt;         // [crt_rows][crt_k]
+    std::vector<uint32_t> q;          // [L]
+    std::vector<uint32_t> neg_p_mod_q;  // [L]  R = (-P) mod q
+    std::vector<uint32_t> corr_t;     // [t_words]
+    std::vector<uint32_t> corr_e;     // [4]
+    // kept for inspection by tests
+    std::vector<uint32_t> m_mod_q;    // [r][L]  M_k = P_k mod q
+    std::vector<uint64_t> reciprocal; // [r]     floor((2^64-1)/p_k)
+    std::vector<uint32_t> beta;       // [r]     operand-b scale folded into its weights
+};
+
+// Validation of the point, as fherma_init does (solve.cu:903-904): N a power of
+// two below 2^29, L == ceil(W/32), q of exactly L limbs' storage, 1 <= q.
+// Throws std::invalid_argument / std::runtime_error with a message.
+HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words);
+
+// fherma_init's prime walk, generalized: descending p = 1 (mod step) from
+// floor((2^31-2)/step)*step + 1, primes only, while prod(p) <= bound, failing
+// below 2^30.  step = 2N reproduces solve.cu:905-913 exactly.
+bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes);
+
+bool is_prime_u32(uint32_t p);
+
+}  // namespace pm

@@ -1,0 +1,212 @@
+"""Python binding of the gfx950 negacyclic multiplier (ctypes over the C ABI of
+include/polymul.h, libpolymul_amd.so in this directory).
+
+The library is the product; this module only marshals numpy arrays and raw
+device pointers.  It never computes a product itself: if the shared library is
+missing, loading fails with an error telling how to build it.
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBRARY_PATH = HERE / "libpolymul_amd.so"
+
+PM_OK = 0
+STATUS_NAMES = {
+    0: "PM_OK",
+    1: "PM_ERR_INVALID_ARGUMENT",
+    2: "PM_ERR_INSUFFICIENT_FIELDS",
+    3: "PM_ERR_DEVICE",
+    4: "PM_ERR_OUT_OF_MEMORY",
+    5: "PM_ERR_INTERNAL",
+    6: "PM_ERR_NO_DEVICE",
+}
+
+TABLE_PRIME_CONSTS = 1
+TABLE_CONV_WEIGHTS = 2
+TABLE_TWIDDLES_FWD = 3
+TABLE_TWIDDLES_INV = 4
+TABLE_CRT_WEIGHTS = 5
+TABLE_M_MOD_Q = 6
+TABLE_NEG_P_MOD_Q = 7
+TABLE_CORR_T = 8
+TABLE_CORR_E = 9
+TABLE_RECIPROCALS = 10
+TABLE_BETA = 11
+TABLE_SHAPE = 12
+
+STAGE_RESIDUES = 1
+STAGE_PRODUCT = 2
+STAGE_WORDS = 3
+STAGE_OUTPUT = 4
+
+SHAPE_FIELDS = ("n", "width", "limbs", "n1", "n2", "log_n2", "r", "r_pad", "conv_k", "conv_rows",
+                "crt_words", "crt_words_pad", "crt_rows", "crt_k", "t_words", "row_width", "rows",
+                "qbits", "s_shift", "k_shift", "mu_lo", "mu_hi", "trivial", "reference_walk")
+
+STAGE_TIME_NAMES = ("conversion_gemm", "ntt_columns_forward", "ntt_rows_product",
+                    "ntt_columns_inverse", "crt_gemm", "reconstruct", "total")
+
+
+class PolymulError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_uint32), ("width", ctypes.c_uint32), ("limbs", ctypes.c_uint32),
+                ("primes", ctypes.c_uint32), ("block", ctypes.c_uint32), ("transform", ctypes.c_uint32),
+                ("row_width", ctypes.c_uint32), ("reference_walk", ctypes.c_uint32),
+                ("qbits", ctypes.c_uint32), ("trivial", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Loads libpolymul_amd.so; raises (never falls back) when it is absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = Path(path) if path else LIBRARY_PATH
+    if not path.is_file():
+        raise FileNotFoundError("%s is not built; run `python __graft_entry__.py` or "
+                                "`python fherma-polynomial-cuda_amd/build.py`" % path)
+    lib = ctypes.CDLL(str(path))
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    vp = ctypes.c_void_p
+    plan_pp = ctypes.POINTER(ctypes.c_void_p)
+    sigs = {
+        "pm_plan_create": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, plan_pp], ctypes.c_int),
+        "pm_plan_create_host": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, plan_pp], ctypes.c_int),
+        "pm_plan_destroy": ([vp], None),
+        "pm_multiply": ([vp, vp, vp, vp], ctypes.c_int),
+        "pm_multiply_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "pm_last_error": ([], ctypes.c_char_p),
+        "pm_plan_get_info": ([vp, ctypes.POINTER(PlanInfo)], ctypes.c_int),
+        "pm_plan_get_primes": ([vp, u32p, ctypes.c_size_t], ctypes.c_int),
+        "pm_plan_export_table": ([vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+        "pm_debug_stage": ([vp, vp, vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+        "pm_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
+        "pm_get_stage_times": ([vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int], ctypes.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path == LIBRARY_PATH:
+        _lib = lib
+    return lib
+
+
+def _check(lib, status):
+    if status != PM_OK:
+        raise PolymulError(status, lib.pm_last_error().decode(errors="replace"))
+
+
+def _u32(array):
+    array = np.ascontiguousarray(array, dtype=np.uint32)
+    return array, array.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+class Plan:
+    """A parameter-only plan (fherma_init) for c = a*b mod (X^N+1, q)."""
+
+    def __init__(self, n, width, limbs, q_words, device=True):
+        self.lib = load_library()
+        q_arr, q_ptr = _u32(q_words)
+        if q_arr.size != limbs:
+            raise ValueError("q must have exactly L=%d limbs" % limbs)
+        handle = ctypes.c_void_p()
+        create = self.lib.pm_plan_create if device else self.lib.pm_plan_create_host
+        _check(self.lib, create(n, width, limbs, q_ptr, ctypes.byref(handle)))
+        self.handle = handle
+        self.n, self.width, self.limbs = n, width, limbs
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.pm_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- inspection -----------------------------------------------------
+    def info(self):
+        info = PlanInfo()
+        _check(self.lib, self.lib.pm_plan_get_info(self.handle, ctypes.byref(info)))
+        return {name: getattr(info, name) for name, _ in PlanInfo._fields_}
+
+    def primes(self):
+        count = self.info()["primes"]
+        out = np.zeros(count, dtype=np.uint32)
+        _check(self.lib, self.lib.pm_plan_get_primes(self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                     count))
+        return out
+
+    def table(self, table_id, dtype):
+        size = ctypes.c_size_t(0)
+        _check(self.lib, self.lib.pm_plan_export_table(self.handle, table_id, None, ctypes.byref(size)))
+        out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(self.lib, self.lib.pm_plan_export_table(self.handle, table_id, out.ctypes.data_as(ctypes.c_void_p),
+                                                       ctypes.byref(size)))
+        return out
+
+    def shape(self):
+        words = self.table(TABLE_SHAPE, np.uint32)
+        s = dict(zip(SHAPE_FIELDS, (int(w) for w in words)))
+        s["mu"] = s.pop("mu_lo") | (s.pop("mu_hi") << 32)
+        return s
+
+    # ---- compute --------------------------------------------------------
+    def multiply(self, a, b):
+        """Host [N][L] uint32 arrays in, host array out (includes transfers)."""
+        a, _ = _u32(a)
+        b, _ = _u32(b)
+        expect = (self.n, self.limbs)
+        if a.shape != expect or b.shape != expect:
+            raise ValueError("operands must be [N][L] = %s" % (expect,))
+        c = np.empty(expect, dtype=np.uint32)
+        _check(self.lib, self.lib.pm_multiply(self.handle, a.ctypes.data_as(ctypes.c_void_p),
+                                              b.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p)))
+        return c
+
+    def multiply_device(self, a_ptr, b_ptr, c_ptr, stream=None):
+        """Device pointers (ints) to [N][L] words; stream is a hipStream_t int or None."""
+        _check(self.lib, self.lib.pm_multiply_device(self.handle, ctypes.c_void_p(a_ptr), ctypes.c_void_p(b_ptr),
+                                                     ctypes.c_void_p(c_ptr),
+                                                     ctypes.c_void_p(stream) if stream else None))
+
+    def debug_stage(self, a, b, stage):
+        a, _ = _u32(a)
+        b, _ = _u32(b)
+        size = ctypes.c_size_t(0)
+        _check(self.lib, self.lib.pm_debug_stage(self.handle, None, None, stage, None, ctypes.byref(size)))
+        dtype = np.int64 if stage == STAGE_WORDS else np.uint32
+        out = np.zeros(size.value // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(self.lib, self.lib.pm_debug_stage(self.handle, a.ctypes.data_as(ctypes.c_void_p),
+                                                 b.ctypes.data_as(ctypes.c_void_p), stage,
+                                                 out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(size)))
+        return out
+
+    def set_profiling(self, enabled=True):
+        _check(self.lib, self.lib.pm_set_profiling(self.handle, 1 if enabled else 0))
+
+    def stage_times(self):
+        out = (ctypes.c_float * len(STAGE_TIME_NAMES))()
+        _check(self.lib, self.lib.pm_get_stage_times(self.handle, out, len(STAGE_TIME_NAMES)))
+        return dict(zip(STAGE_TIME_NAMES, (float(x) for x in out)))

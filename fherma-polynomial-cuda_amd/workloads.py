@@ -1,0 +1,103 @@
+"""Seeded synthetic operands for the multiplier's workloads (numpy only).
+
+Configurations (limb counts L; W = 32L bits of q unless stated):
+  small   N=1024,  L=8     random odd 256-bit q, uniform operands
+  mid     N=32768, L=868   random odd 27,776-bit q, uniform operands with 1% of
+                           coefficients forced to q-1 and 1% to 0
+  large   N=65536, L=1747  random odd 55,904-bit q, uniform operands
+The reference's own competition points (q the NTT-friendly prime below 2^W,
+W = 1747 or 868 bits) take q from tests/golden/points.json.
+
+q is 32L random bits with the top bit and bit 0 set; coefficients are uniform
+in [0, q) by rejection.  numpy's PCG64 with a fixed seed makes every case
+reproducible on any machine with the same numpy.
+"""
+import numpy as np
+
+CONFIGS = {
+    "small": {"n": 1024, "limbs": 8, "seed": 0x5EED0001},
+    "mid": {"n": 32768, "limbs": 868, "seed": 0x5EED0002},
+    "large": {"n": 65536, "limbs": 1747, "seed": 0x5EED0003},
+}
+
+
+def words_to_int(words):
+    return int.from_bytes(np.ascontiguousarray(words, dtype="<u4").tobytes(), "little")
+
+
+def int_to_words(value, limbs):
+    return np.frombuffer(int(value).to_bytes(4 * limbs, "little"), dtype="<u4").astype(np.uint32)
+
+
+def random_modulus(limbs, rng):
+    """32L random bits, top bit and bit 0 set."""
+    q = rng.integers(0, 1 << 32, size=limbs, dtype=np.uint64).astype(np.uint32)
+    q[-1] |= np.uint32(0x80000000)
+    q[0] |= np.uint32(1)
+    return q
+
+
+def _ge_rows(values, q_words):
+    """Boolean per row: value >= q (lexicographic from the top limb)."""
+    rev = values[:, ::-1]
+    qrev = q_words[::-1]
+    diff = rev != qrev
+    has = diff.any(axis=1)
+    first = np.argmax(diff, axis=1)
+    rows = np.arange(values.shape[0])
+    return ~has | (rev[rows, first] > qrev[first])
+
+
+def uniform_below(q_words, n, rng):
+    """[n][L] uint32, each row uniform in [0, q)."""
+    q_words = np.ascontiguousarray(q_words, dtype=np.uint32)
+    limbs = q_words.size
+    top = int(q_words[-1])
+    if top == 0:
+        raise ValueError("q's top limb must be non-zero")
+    mask = np.uint32((1 << top.bit_length()) - 1)
+    out = np.empty((n, limbs), dtype=np.uint32)
+    pending = np.arange(n)
+    while pending.size:
+        draw = rng.integers(0, 1 << 32, size=(pending.size, limbs), dtype=np.uint64).astype(np.uint32)
+        draw[:, -1] &= mask
+        bad = _ge_rows(draw, q_words)
+        good = ~bad
+        out[pending[good]] = draw[good]
+        pending = pending[bad]
+    return out
+
+
+def q_minus_one(q_words):
+    q = np.array(q_words, dtype=np.uint32, copy=True)
+    i = 0
+    while q[i] == 0:
+        q[i] = 0xFFFFFFFF
+        i += 1
+    q[i] -= np.uint32(1)
+    return q
+
+
+def baseline_case(name, seed=None):
+    """(n, width_bits, limbs, q_words, a, b) for one of CONFIGS."""
+    cfg = CONFIGS[name]
+    n, limbs = cfg["n"], cfg["limbs"]
+    rng = np.random.Generator(np.random.PCG64(cfg["seed"] if seed is None else seed))
+    q = random_modulus(limbs, rng)
+    a = uniform_below(q, n, rng)
+    b = uniform_below(q, n, rng)
+    if name == "mid":
+        qm1 = q_minus_one(q)
+        for arr in (a, b):
+            idx = rng.permutation(n)
+            k = n // 100
+            arr[idx[:k]] = qm1
+            arr[idx[k:2 * k]] = 0
+    return n, 32 * limbs, limbs, q, a, b
+
+
+def operands_for(n, q_words, seed):
+    """Two uniform [n][L] operands below q from one seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    q_words = np.ascontiguousarray(q_words, dtype=np.uint32)
+    return uniform_below(q_words, n, rng), uniform_below(q_words, n, rng)

@@ -1,0 +1,131 @@
+/*
+ * C ABI of the MI355X (gfx950) exact negacyclic multiplier
+ *     c = a * b  mod (X^N + 1, q)
+ * over canonical little-endian 32-bit limb arrays (coefficient-major [N][L]).
+ *
+ * This is the thin layer below the drop-in C++ surface of include/fherma.h.
+ * The reference exposes only that C++ surface (src/fherma.h:46-48, with
+ * std::vector-backed types and exceptions); the functions here carry the same
+ * three roles with plain pointers and status codes, and the fherma shim
+ * (fherma-polynomial-cuda_amd/csrc/fherma_shim.cpp) turns codes into
+ * std::runtime_error exactly where the reference throws.
+ *
+ * Threading: a plan may be used by one thread at a time.  Every entry point
+ * returns only after its device work has finished, except
+ * pm_multiply_device_async, which returns once the work is queued.
+ */
+#ifndef POLYMUL_AMD_H
+#define POLYMUL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pm_status {
+    PM_OK = 0,
+    PM_ERR_INVALID_ARGUMENT = 1,  /* bad point or input size (solve.cu:903-904, 1228) */
+    PM_ERR_INSUFFICIENT_FIELDS = 2, /* no prime family covers 4N(q-1)^2 (solve.cu:910) */
+    PM_ERR_DEVICE = 3,            /* a HIP call failed (solve.cu:80-82) */
+    PM_ERR_OUT_OF_MEMORY = 4,
+    PM_ERR_INTERNAL = 5,
+    PM_ERR_NO_DEVICE = 6          /* compute requested but no GPU / no device state */
+} pm_status;
+
+typedef struct pm_plan pm_plan;
+
+typedef struct pm_plan_info {
+    uint32_t n, width, limbs;
+    uint32_t primes;        /* r: number of ~31-bit NTT fields */
+    uint32_t block;         /* n1: transform split N = n1 * n2 (1 = plain negacyclic NTT) */
+    uint32_t transform;     /* n2 */
+    uint32_t row_width;     /* NTT row length C */
+    uint32_t reference_walk;/* 1 when the primes are exactly fherma_init's walk */
+    uint32_t qbits;
+    uint32_t trivial;       /* 1 when q == 1 */
+    uint64_t device_bytes;  /* device memory held by the plan (0 for host-only plans) */
+} pm_plan_info;
+
+/* Parameter-only initialization: replaces fherma_init (src/fherma.h:46;
+ * candidates/ordered-streaming/solve.cu:900-1040).  width = bits of q,
+ * limbs = ceil(width/32), q = limbs little-endian words.  Builds the prime
+ * family and every table on the host, uploads them to the current device and
+ * allocates all working memory.  No operand-dependent work happens here. */
+pm_status pm_plan_create(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q,
+                         pm_plan** out);
+
+/* Same plan, host tables only (no device, no HIP calls): for inspecting the
+ * parameter-only plan on a machine without a GPU.  Compute entry points
+ * return PM_ERR_NO_DEVICE on such a plan. */
+pm_status pm_plan_create_host(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q,
+                              pm_plan** out);
+
+/* Releases everything the plan holds: replaces fherma_free (src/fherma.h:48). */
+void pm_plan_destroy(pm_plan* plan);
+
+/* Host vectors in, host vector out, one product: replaces fherma_run
+ * (src/fherma.h:47; candidates/ordered-streaming/solve.cu:1226-1271).  a, b, c
+ * are [N][L] words; c may not alias a or b.  Includes the host->device and
+ * device->host transfers, like the reference's timed window. */
+pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c);
+
+/* Device-resident product: a, b, c are device pointers to [N][L] words on the
+ * plan's device; stream is a hipStream_t (NULL = the plan's own stream).
+ * No counterpart in the reference (its operands always arrive in host
+ * vectors); it is the kernel-only form of fherma_run for callers whose
+ * operands already live in HBM.  Synchronizes the stream before returning. */
+pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                             void* stream);
+
+/* Message describing the most recent failure on the calling thread. */
+const char* pm_last_error(void);
+
+pm_status pm_plan_get_info(const pm_plan* plan, pm_plan_info* info);
+
+/* Copies min(capacity, r) primes; returns PM_OK. */
+pm_status pm_plan_get_primes(const pm_plan* plan, uint32_t* primes, size_t capacity);
+
+/* Host tables of the plan, for tests.  Identifiers: */
+enum {
+    PM_TABLE_PRIME_CONSTS = 1, /* [r] x {p, p^-1 mod 2^32, off_a, off_b} u32 */
+    PM_TABLE_CONV_WEIGHTS = 2, /* [2][4*r_pad][conv_k] int8 */
+    PM_TABLE_TWIDDLES_FWD = 3, /* [r][n2][2] u32 */
+    PM_TABLE_TWIDDLES_INV = 4, /* [r][n2][2] u32 */
+    PM_TABLE_CRT_WEIGHTS = 5,  /* [4*crt_words_pad][4*r_pad] int8 */
+    PM_TABLE_M_MOD_Q = 6,      /* [r][L] u32: M_k = P_k mod q */
+    PM_TABLE_NEG_P_MOD_Q = 7,  /* [L] u32: (-P) mod q */
+    PM_TABLE_CORR_T = 8,       /* [L+2] u32 */
+    PM_TABLE_CORR_E = 9,       /* [4] u32 */
+    PM_TABLE_RECIPROCALS = 10, /* [r] u64: floor((2^64-1)/p_k) */
+    PM_TABLE_BETA = 11,        /* [r] u32: operand-b scale */
+    PM_TABLE_SHAPE = 12        /* 24 x u32: see pm_plan_shape_words */
+};
+/* Returns the byte size of a table through *bytes when out == NULL. */
+pm_status pm_plan_export_table(const pm_plan* plan, int table, void* out, size_t* bytes);
+
+/* Per-stage parity taps (GPU): runs the pipeline on host inputs a, b and
+ * copies out the buffer produced by `stage`:
+ *   PM_STAGE_RESIDUES  [2][r][N] u32  a mod p_k and beta_k * b mod p_k
+ *   PM_STAGE_PRODUCT   [r][N]    u32  v_k = x * P_k^-1 mod p_k (x = exact
+ *                                      signed negacyclic integer product)
+ *   PM_STAGE_WORDS     [L+4][N]  i64  CRT word sums (signed-byte form)
+ *   PM_STAGE_OUTPUT    [N][L]    u32  c
+ * Returns the needed size through *bytes when out == NULL. */
+enum { PM_STAGE_RESIDUES = 1, PM_STAGE_PRODUCT = 2, PM_STAGE_WORDS = 3, PM_STAGE_OUTPUT = 4 };
+pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, int stage, void* out,
+                         size_t* bytes);
+
+/* Kernel timing of the most recent device multiply, when enabled: fills up to
+ * `count` entries of ms (0: conversion GEMM, 1: forward column NTT,
+ * 2: row NTT + product + inverse row NTT, 3: inverse column NTT, 4: CRT GEMM,
+ * 5: reconstruction, 6: total) measured with HIP events on the plan's stream. */
+pm_status pm_set_profiling(pm_plan* plan, int enabled);
+pm_status pm_get_stage_times(const pm_plan* plan, float* ms, int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POLYMUL_AMD_H */

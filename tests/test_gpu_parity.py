@@ -1,0 +1,195 @@
+"""GPU parity of the gfx950 path against the CPU oracle (bit-exact).
+
+Every product goes through the C ABI of libpolymul_amd.so; the oracle is
+oracle/reference.py (GMP Kronecker, tests/reference.py:29-43 restated) and,
+per stage, the closed forms the oracle pins:
+  residues   a mod p_k, beta_k*b mod p_k              (conversion GEMM)
+  product    x_i * P_k^-1 mod p_k, x = exact signed   (NTTs + base case)
+             negacyclic integer product
+  words      sum_j 2^32j D_j + corr == sum_k v_k M_k   (CRT GEMM, exact)
+  output     c == Kronecker oracle
+"""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(*what):
+    print("[gpu-case]", *what, flush=True)
+    sys.stdout.flush()
+
+
+def _limbs(values, limbs, oracle):
+    return oracle.to_limbs(values, limbs)
+
+
+def _check_product(plan, a, b, q_words, oracle):
+    c = plan.multiply(a, b)
+    expect = oracle.kronecker_gmp(a, b, q_words)
+    mismatch = np.nonzero((c != expect).any(axis=1))[0]
+    assert mismatch.size == 0, "coefficients differ, first at %s of %d" % (mismatch[:5], mismatch.size)
+
+
+# ---------------------------------------------------------------------------
+# per-stage parity at small sizes
+# ---------------------------------------------------------------------------
+STAGE_POINTS = [
+    (8, 16, "ntt"),        # tiny
+    (64, 33, "ntt"),       # two limbs
+    (1024, 256, "random"),  # the small plain config
+]
+
+
+@pytest.mark.parametrize("n,width,kind", STAGE_POINTS)
+def test_stages(polymul, workloads, oracle, n, width, kind):
+    _announce("stages", n, width, kind)
+    limbs = (width + 31) // 32
+    if kind == "ntt":
+        q = oracle.modulus(n, width)
+    else:
+        rng = np.random.Generator(np.random.PCG64(7))
+        q = oracle.words_to_int(workloads.random_modulus(limbs, rng))
+    q_words = oracle.int_to_words(q, limbs)
+    a_list, b_list = oracle.random_pair(n, q, 101)
+    a, b = oracle.to_limbs(a_list, limbs), oracle.to_limbs(b_list, limbs)
+    with polymul.Plan(n, width, limbs, q_words) as plan:
+        primes = [int(p) for p in plan.primes()]
+        r = len(primes)
+        beta = [int(x) for x in plan.table(polymul.TABLE_BETA, np.uint32)]
+        # 1. conversion
+        res = plan.debug_stage(a, b, polymul.STAGE_RESIDUES).reshape(2, r, n)
+        for k, p in enumerate(primes):
+            assert [int(v) for v in res[0, k]] == [x % p for x in a_list], "a mod p_%d" % k
+            assert [int(v) for v in res[1, k]] == [beta[k] * x % p for x in b_list], "beta*b mod p_%d" % k
+        # 2. transforms + base case + inverse
+        x = oracle.signed_convolution(a_list, b_list)
+        big_p = 1
+        for p in primes:
+            big_p *= p
+        prod = plan.debug_stage(a, b, polymul.STAGE_PRODUCT).reshape(r, n)
+        for k, p in enumerate(primes):
+            inv = pow((big_p // p) % p, -1, p)
+            assert [int(v) for v in prod[k]] == [(xi % p) * inv % p for xi in x], "v_%d" % k
+        # 3. CRT word sums: exact identities of the signed-byte GEMM
+        words = plan.debug_stage(a, b, polymul.STAGE_WORDS).reshape(limbs + 4, n)
+        m_mod_q = plan.table(polymul.TABLE_M_MOD_Q, np.uint32).reshape(r, limbs)
+        m_vals = [oracle.words_to_int(m_mod_q[k]) for k in range(r)]
+        rcp = [int(v) for v in plan.table(polymul.TABLE_RECIPROCALS, np.uint64)]
+        corr_t = oracle.words_to_int(plan.table(polymul.TABLE_CORR_T, np.uint32))
+        corr_e = oracle.words_to_int(plan.table(polymul.TABLE_CORR_E, np.uint32))
+        for i in range(0, n, max(1, n // 16)):
+            v = [int(prod[k, i]) for k in range(r)]
+            t_sum = sum(int(words[j, i]) << (32 * j) for j in range(limbs + 1)) + corr_t
+            assert t_sum == sum(vk * mk for vk, mk in zip(v, m_vals)), "T sum at %d" % i
+            e_sum = sum(int(words[limbs + 1 + j, i]) << (32 * j) for j in range(3)) + corr_e
+            assert e_sum == sum(vk * rk for vk, rk in zip(v, rcp)), "E sum at %d" % i
+            _, quotient, _ = oracle.crt_reconstruct(x[i], primes, q)
+            assert (e_sum + (1 << 63)) >> 64 == quotient
+        # 4. output
+        c = plan.debug_stage(a, b, polymul.STAGE_OUTPUT).reshape(n, limbs)
+        assert oracle.from_limbs(c) == [xi % q for xi in x]
+
+
+# ---------------------------------------------------------------------------
+# the reference's validation matrix (tools/validate_local.py:13-14), NTT-prime q,
+# edge cases of tests/reference.py:52-57 and seeds 101, 202
+# ---------------------------------------------------------------------------
+VALIDATE_POINTS = [(1, 2), (2, 5), (8, 16), (64, 31), (64, 33), (128, 128), (256, 256),
+                   (1024, 512), (8192, 256), (32768, 868), (65536, 1747)]
+
+
+@pytest.mark.parametrize("n,width", VALIDATE_POINTS)
+def test_reference_matrix(polymul, oracle, n, width):
+    _announce("reference-matrix", n, width)
+    limbs = (width + 31) // 32
+    q = oracle.modulus(n, width)
+    q_words = oracle.int_to_words(q, limbs)
+    pairs = oracle.edge_pairs(n, q) + [oracle.random_pair(n, q, s) for s in (101, 202)]
+    with polymul.Plan(n, width, limbs, q_words) as plan:
+        for a_list, b_list in pairs:
+            a, b = oracle.to_limbs(a_list, limbs), oracle.to_limbs(b_list, limbs)
+            _check_product(plan, a, b, q_words, oracle)
+
+
+# tools/stress_local.py:16-27 boundary pairs (wider coefficients, larger transform)
+STRESS_POINTS = [(128, 4096), (131072, 128), (32768, 868), (65536, 1747)]
+
+
+@pytest.mark.parametrize("n,width", STRESS_POINTS)
+def test_stress_boundaries(polymul, oracle, n, width):
+    _announce("stress", n, width)
+    limbs = (width + 31) // 32
+    q = oracle.modulus(n, width)
+    q_words = oracle.int_to_words(q, limbs)
+    pairs = oracle.stress_pairs(n, q, limbs) + [oracle.random_pair(n, q, 10001)]
+    with polymul.Plan(n, width, limbs, q_words) as plan:
+        for a_list, b_list in pairs:
+            a, b = oracle.to_limbs(a_list, limbs), oracle.to_limbs(b_list, limbs)
+            _check_product(plan, a, b, q_words, oracle)
+
+
+# ---------------------------------------------------------------------------
+# the plain configurations with random odd q
+# ---------------------------------------------------------------------------
+def _edge_operands(workloads, q_words, n, kind):
+    limbs = q_words.size
+    z = np.zeros((n, limbs), dtype=np.uint32)
+    if kind == "zero":
+        return z
+    if kind == "unit":
+        z[0, 0] = 1
+        return z
+    if kind == "max":
+        return np.tile(workloads.q_minus_one(q_words), (n, 1))
+    if kind.startswith("x^"):
+        z[int(kind[2:]), 0] = 1
+        return z
+    raise ValueError(kind)
+
+
+def test_small_random_modulus(polymul, workloads, oracle):
+    _announce("small")
+    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        assert plan.info()["primes"] == 17
+        _check_product(plan, a, b, q, oracle)
+        for kind in ("zero", "unit", "max", "x^0", "x^1", "x^%d" % (n - 1)):
+            e = _edge_operands(workloads, q, n, kind)
+            _check_product(plan, e, b, q, oracle)
+            _check_product(plan, a, e, q, oracle)
+        m = _edge_operands(workloads, q, n, "max")
+        _check_product(plan, m, m, q, oracle)
+
+
+def test_mid_random_modulus(polymul, workloads, oracle):
+    _announce("mid")
+    n, width, limbs, q, a, b = workloads.baseline_case("mid")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        info = plan.info()
+        assert info["block"] > 1  # the split transform is needed here
+        _check_product(plan, a, b, q, oracle)
+
+
+def test_large_random_modulus_properties(polymul, workloads, oracle):
+    """The large config at full size: unit and shift identities, commutativity,
+    and sampled coefficients against the oracle's direct signed sum."""
+    _announce("large")
+    n, width, limbs, q, a, b = workloads.baseline_case("large")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        c = plan.multiply(a, b)
+        for i in (0, 1, n // 2, n - 1):
+            _announce("large coefficient", i)
+            assert np.array_equal(c[i], oracle.coefficient_gmp(a, b, q, i)), "coefficient %d" % i
+        unit = _edge_operands(workloads, q, n, "unit")
+        assert np.array_equal(plan.multiply(a, unit), a)
+        # X * a: coefficients shift up by one, the top one wraps with a sign flip
+        x1 = _edge_operands(workloads, q, n, "x^1")
+        shifted = plan.multiply(x1, a)
+        assert np.array_equal(shifted[1:], a[:-1])
+        top = oracle.words_to_int(a[-1])
+        assert oracle.words_to_int(shifted[0]) == (-top) % oracle.words_to_int(q)
+        assert np.array_equal(plan.multiply(b, a), c)
