@@ -1,0 +1,201 @@
+"""Benchmark of the gfx950 exact negacyclic multiplier.
+
+Workload (one "step" = one full product, device-resident operands):
+  N = 65536, L = 1747 limbs: q a seeded random odd 55,904-bit modulus, a and b
+  uniform in [0, q) (458 MB each), c = a*b mod (X^N + 1, q), bit-exact.
+
+Prints ONE JSON line with ms/product (lower is better), the live HIP-event
+timing of the dominant kernel (the conversion GEMM on INT8 MFMA) as a
+roofline fraction, the transfer-inclusive time through the host-vector entry
+point, the reference's own competition point (N=65536, 1747-bit NTT-prime q,
+55 limbs) and a bounded CPU GMP baseline run on this host.
+"""
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG = ROOT / "fherma-polynomial-cuda_amd"
+METRIC = "ms/product at N=65536,W=1747 (bit-exact); batched products/s vs plain; p99"
+INT8_PEAK_TOPS = 5000.0  # MI355X dense INT8 MFMA, TOP/s
+HBM_PEAK_GBS = 8000.0
+
+
+def _load(name):
+    spec = importlib.util.spec_from_file_location("pm_" + name, PKG / (name + ".py"))
+    module = importlib.util.module_from_spec(spec)
+    sys.modules["pm_" + name] = module
+    spec.loader.exec_module(module)
+    return module
+
+
+def conversion_ops(n, limbs, r):
+    """Algorithmic INT8 ops of one conversion launch: both operands, 2 ops per
+    byte MAC, unpadded K = 4L bytes, 4 digit rows per prime."""
+    return 2 * 2 * n * (4 * limbs) * (4 * r)
+
+
+def crt_ops(n, limbs, r):
+    """Algorithmic INT8 ops of one CRT launch: K = 4r residue bytes, 4(L+2) output bytes."""
+    return 2 * n * (4 * r) * (4 * (limbs + 2))
+
+
+def time_device(plan, torch, a_np, b_np, steps, warmup):
+    n, limbs = a_np.shape
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(a_np.view(np.int32)).to(dev)
+    b = torch.from_numpy(b_np.view(np.int32)).to(dev)
+    c = torch.empty((n, limbs), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    plan.set_profiling(True)
+    for _ in range(warmup):
+        plan.multiply_device(a.data_ptr(), b.data_ptr(), c.data_ptr())
+    torch.cuda.synchronize()
+    stage_sum = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.multiply_device(a.data_ptr(), b.data_ptr(), c.data_ptr())
+        times = plan.stage_times()
+        stage_sum = {k: v + (stage_sum or {}).get(k, 0.0) for k, v in times.items()}
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stage_avg = {k: v / steps for k, v in stage_sum.items()}
+    out = c.cpu().numpy().view(np.uint32)
+    del a, b, c
+    return elapsed * 1000.0 / steps, stage_avg, out
+
+
+def cpu_baseline(q, a, b, plan_cls, sample_n):
+    """The oracle's GMP Kronecker product (one core) on a bounded sample: the
+    first sample_n coefficients of each operand as an N=sample_n product with
+    the same q; the GPU computes the same sample for an exact comparison."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import reference
+
+    sa = np.ascontiguousarray(a[:sample_n])
+    sb = np.ascontiguousarray(b[:sample_n])
+    t0 = time.perf_counter()
+    expect = reference.kronecker_gmp(sa, sb, q)
+    cpu_s = time.perf_counter() - t0
+    limbs = q.size
+    with plan_cls(sample_n, 32 * limbs, limbs, q) as plan:
+        got = plan.multiply(sa, sb)
+    exact = bool(np.array_equal(got, expect))
+    # GMP's product of Kronecker integers of B bits costs ~B log B; scale the
+    # sample (sample_n coefficients) to the full N by that law.
+    bits_sample = sample_n * ((sample_n * (reference.words_to_int(q) - 1) ** 2).bit_length())
+    n_full = a.shape[0]
+    bits_full = n_full * ((n_full * (reference.words_to_int(q) - 1) ** 2).bit_length())
+    scale = (bits_full * math.log2(bits_full)) / (bits_sample * math.log2(bits_sample))
+    return cpu_s, scale, exact, reference
+
+
+def main():
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    parser.add_argument("--no-secondary", action="store_true", help="skip the reference-point and e2e legs")
+    parser.add_argument("--cpu-sample", type=int, default=8192)
+    args = parser.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: one product runs on one GPU; the multi-GPU "
+              "sharding (by primes, one reduce-scatter of CRT word sums) is a design, not a "
+              "measured path. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    polymul = _load("polymul")
+    workloads = _load("workloads")
+    t_gen = time.perf_counter()
+    n, width, limbs, q, a, b = workloads.baseline_case("large")
+    gen_s = time.perf_counter() - t_gen
+    t_plan = time.perf_counter()
+    plan = polymul.Plan(n, width, limbs, q)
+    plan_s = time.perf_counter() - t_plan
+    info = plan.info()
+    r = info["primes"]
+
+    ms, stages, c = time_device(plan, torch, a, b, args.steps, args.warmup)
+
+    result = {
+        "metric": METRIC,
+        "value": round(ms, 4),
+        "unit": "ms/product",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded uniform operands below a seeded random odd q)",
+        "config": {"workload": "plain product, N=65536, L=1747 limbs (55,904-bit random odd q), "
+                               "operands resident in HBM",
+                   "n": n, "limbs": limbs, "primes": r, "split": info["block"], "transform": info["transform"],
+                   "parallelism": "single GPU"},
+    }
+    conv_ms = stages["conversion_gemm"]
+    ops = conversion_ops(n, limbs, r)
+    achieved = ops / (conv_ms * 1e-3) / 1e12
+    result["roofline"] = {"kernel": "conversion_gemm", "bound": "mfma", "achieved": round(achieved, 1),
+                          "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                          "traffic": None, "ops_per_launch": ops, "avg_launch_ms": round(conv_ms, 4)}
+    result["stages_ms"] = {k: round(v, 4) for k, v in stages.items()}
+    result["crt_gemm_tops"] = round(crt_ops(n, limbs, r) / (stages["crt_gemm"] * 1e-3) / 1e12, 1)
+    result["setup_s"] = {"inputs": round(gen_s, 2), "plan": round(plan_s, 2)}
+
+    if not args.no_secondary:
+        # transfer-inclusive: host vectors in, host vector out (pm_multiply)
+        plan.set_profiling(False)
+        t0 = time.perf_counter()
+        c_host = plan.multiply(a, b)
+        result["e2e_host_ms"] = round((time.perf_counter() - t0) * 1000.0, 3)
+        result["e2e_matches_device"] = bool(np.array_equal(c_host, c))
+        del c_host
+    plan.close()
+
+    if not args.no_secondary:
+        points = json.loads((ROOT / "tests" / "golden" / "points.json").read_text())
+        ref = points["65536/1747"]
+        ref_q = workloads.int_to_words(int(ref["q"], 16), ref["L"])
+        ra, rb = workloads.operands_for(ref["N"], ref_q, 101)
+        with polymul.Plan(ref["N"], ref["W"], ref["L"], ref_q) as rplan:
+            rms, rstages, _ = time_device(rplan, torch, ra, rb, max(args.steps, 10), args.warmup)
+            t0 = time.perf_counter()
+            rplan.set_profiling(False)
+            rplan.multiply(ra, rb)
+            r_e2e = (time.perf_counter() - t0) * 1000.0
+        result["reference_point"] = {"N": 65536, "W_bits": 1747, "L": 55, "ms_device": round(rms, 4),
+                                     "ms_e2e_host": round(r_e2e, 3),
+                                     "published_ms_e2e_rtx_pro_6000": 1.0712435}
+
+    if not args.no_cpu:
+        cpu_s, scale, exact, reference = cpu_baseline(q, a, b, polymul.Plan, args.cpu_sample)
+        result["cpu_baseline"] = {"value": round(cpu_s * scale * 1000.0, 1), "unit": "ms/product", "cores": 1,
+                                  "kind": "port",
+                                  "sample": "GMP Kronecker product (oracle, 1 core) of the first %d coefficients "
+                                            "as an N=%d product with the same q: %.2f s, scaled to N=%d by "
+                                            "B*log2(B) of the packed integer sizes (x%.2f)"
+                                            % (args.cpu_sample, args.cpu_sample, cpu_s, n, scale),
+                                  "sample_gpu_exact": exact}
+        # sampled bit-exactness of the timed full-size product (checker only)
+        checks = []
+        for i in (0, n - 1):
+            checks.append(bool(np.array_equal(c[i], reference.coefficient_gmp(a, b, q, i))))
+        result["exact_sampled_coefficients"] = all(checks)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()
