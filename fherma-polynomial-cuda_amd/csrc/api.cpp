@@ -33,7 +33,9 @@ void hip_check(hipError_t e, const char* where) {
     if (e != hipSuccess) throw DeviceError(std::string(where) + ": " + hipGetErrorString(e));
 }
 
-constexpr int kStages = 7;
+// Events bracket the 7 stage launches; times are the 7 intervals plus the total.
+constexpr int kEvents = 8;
+constexpr int kStages = 8;
 
 }  // namespace
 
@@ -50,7 +52,7 @@ struct pm_plan {
     uint32_t* io_c = nullptr;
     uint64_t device_bytes = 0;
     bool profiling = false;
-    hipEvent_t events[kStages] = {};
+    hipEvent_t events[kEvents] = {};
     float stage_ms[kStages] = {};
 
     template <class T>
@@ -97,22 +99,35 @@ struct pm_plan {
         io_b = alloc<uint32_t>(nl);
         io_c = alloc<uint32_t>(nl);
         if (s.trivial) return;
+        pm::configure_kernels();
         tables.consts = upload(host.consts);
-        tables.wconv = upload(host.wconv);
+        {
+            // both operands' weight matrices, each in 256x64 tiles
+            const size_t one = size_t(s.conv_rows) * s.conv_k;
+            std::vector<int8_t> tiled;
+            tiled.reserve(2 * one);
+            for (int op = 0; op < 2; ++op) {
+                std::vector<int8_t> part(host.wconv.begin() + op * one, host.wconv.begin() + (op + 1) * one);
+                std::vector<int8_t> t = pm::tile_256x64(part, s.conv_rows, s.conv_k);
+                tiled.insert(tiled.end(), t.begin(), t.end());
+            }
+            tables.wconv = upload(tiled);
+        }
         tables.tw_fwd = reinterpret_cast<const uint2*>(upload(host.tw_fwd));
         tables.tw_inv = reinterpret_cast<const uint2*>(upload(host.tw_inv));
-        tables.wcrt = upload(host.wcrt);
+        tables.wcrt = upload(pm::tile_256x64(host.wcrt, s.crt_rows, s.crt_k));
         tables.q = upload(host.q);
         tables.neg_p = upload(host.neg_p_mod_q);
         tables.corr_t = upload(host.corr_t);
         tables.corr_e = upload(host.corr_e);
-        // planes: [2][r][N]; after the row kernel the second plane is free and
-        // holds the reconstruction's T scratch ([L+2][N]).
-        size_t plane = size_t(s.r) * s.n;
-        size_t scratch = size_t(s.t_words) * s.n;
+        work.packed = alloc<int8_t>(pm::packed_operand_bytes(s));
+        // planes: [2][r][n_pad]; after the row kernel the second plane is free
+        // and holds the reconstruction's T scratch ([L+2][n_pad]).
+        size_t plane = size_t(s.r) * s.n_pad;
+        size_t scratch = size_t(s.t_words) * s.n_pad;
         work.planes = alloc<uint32_t>(plane + (plane > scratch ? plane : scratch));
         work.t_words = work.planes + plane;
-        work.words = alloc<int64_t>(size_t(s.crt_words) * s.n);
+        work.words = alloc<int64_t>(size_t(s.crt_words) * s.n_pad);
     }
 
     // Runs stages up to and including `last` (PM_STAGE_*) on device operands.
@@ -126,28 +141,30 @@ struct pm_plan {
             if (profiling) hip_check(hipEventRecord(events[i], st), "hipEventRecord");
         };
         mark(0);
-        pm::launch_conversion(s, tables, a, b, work.planes, st);
+        pm::launch_pack(s, a, b, work.packed, st);
         mark(1);
+        pm::launch_conversion(s, tables, work.packed, work.planes, st);
+        mark(2);
         if (last == PM_STAGE_RESIDUES) return;
         pm::launch_ntt_forward_columns(s, tables, work.planes, st);
-        mark(2);
-        pm::launch_ntt_rows_product(s, tables, work.planes, st);
         mark(3);
-        pm::launch_ntt_inverse_columns(s, tables, work.planes, st);
+        pm::launch_ntt_rows_product(s, tables, work.planes, st);
         mark(4);
+        pm::launch_ntt_inverse_columns(s, tables, work.planes, st);
+        mark(5);
         if (last == PM_STAGE_PRODUCT) return;
         pm::launch_crt(s, tables, work.planes, work.words, st);
-        mark(5);
+        mark(6);
         if (last == PM_STAGE_WORDS) return;
         pm::launch_reconstruct(s, tables, work.words, work.t_words, c, st);
-        mark(6);
+        mark(7);
     }
 
     void collect_times() {
         if (!profiling || host.shape.trivial) return;
-        for (int i = 0; i < kStages - 1; ++i)
+        for (int i = 0; i < kEvents - 1; ++i)
             hip_check(hipEventElapsedTime(&stage_ms[i], events[i], events[i + 1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&stage_ms[kStages - 1], events[0], events[kStages - 1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&stage_ms[kEvents - 1], events[0], events[kEvents - 1]), "hipEventElapsedTime");
     }
 };
 
@@ -337,10 +354,20 @@ pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, in
         hip_check(hipMemcpyAsync(plan->io_a, a, nl, hipMemcpyHostToDevice, st), "copy a");
         hip_check(hipMemcpyAsync(plan->io_b, b, nl, hipMemcpyHostToDevice, st), "copy b");
         plan->run(plan->io_a, plan->io_b, plan->io_c, stage, st);
-        const void* src = stage == PM_STAGE_WORDS ? static_cast<const void*>(plan->work.words)
-                          : stage == PM_STAGE_OUTPUT ? static_cast<const void*>(plan->io_c)
-                                                     : static_cast<const void*>(plan->work.planes);
-        hip_check(hipMemcpyAsync(out, src, need, hipMemcpyDeviceToHost, st), "copy stage");
+        if (stage == PM_STAGE_OUTPUT) {
+            hip_check(hipMemcpyAsync(out, plan->io_c, need, hipMemcpyDeviceToHost, st), "copy stage");
+        } else {
+            // device rows have stride n_pad; the tap returns rows of exactly N
+            const size_t elem = stage == PM_STAGE_WORDS ? 8 : 4;
+            const size_t rows = stage == PM_STAGE_WORDS ? s.crt_words
+                                : stage == PM_STAGE_RESIDUES ? 2 * size_t(s.r)
+                                                             : size_t(s.r);
+            const void* src = stage == PM_STAGE_WORDS ? static_cast<const void*>(plan->work.words)
+                                                      : static_cast<const void*>(plan->work.planes);
+            hip_check(hipMemcpy2DAsync(out, s.n * elem, src, size_t(s.n_pad) * elem, s.n * elem, rows,
+                                       hipMemcpyDeviceToHost, st),
+                      "copy stage");
+        }
         hip_check(hipStreamSynchronize(st), "debug stage");
         *bytes = need;
     });

@@ -1,6 +1,7 @@
 // gfx950 kernels of the exact negacyclic multiplier c = a*b mod (X^N+1, q).
 //
 // Pipeline (per product; every stage is integer-only):
+//   0. pack_operands        limbs -> 64-byte-aligned K tiles of signed bytes
 //   1. conversion_gemm      limb bytes x signed byte weights on INT8 MFMA
 //                           (v_mfma_i32_16x16x64_i8), REDC -> residues mod p_k
 //                           [reference: tensor_product<true>, solve.cu:143-250]
@@ -22,6 +23,9 @@
 // Signed INT8 MFMA: data bytes u in [0,255] enter as u-128 (XOR 0x80); the
 // weights are signed base-256 digits; the constant 128*sum(weights) is folded
 // into per-prime offsets (conversion) and a fixed big-integer correction (CRT).
+//
+// Planes, word sums and scratch use a row stride of n_pad = N rounded up to
+// 256, so every GEMM tile load is in bounds and 16-byte aligned.
 #include "kernels.hpp"
 
 #include <stdexcept>
@@ -31,11 +35,19 @@ namespace pm {
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 
-constexpr uint32_t kBM = 128;     // GEMM rows per block (weight rows)
-constexpr uint32_t kBN = 128;     // GEMM columns per block (coefficients)
-constexpr uint32_t kBK = 64;      // bytes of K per MFMA step
-constexpr uint32_t kGemmThreads = 256;
+// GEMM block tile: 256 weight rows x 256 coefficients x 64 bytes of K, eight
+// waves in a 2x4 grid, each owning a 128x64 slab = 8x4 MFMA 16x16 tiles
+// (128 int32 accumulators per lane; two waves per SIMD).
+constexpr uint32_t kTile = 256;
+constexpr uint32_t kTK = 64;
+constexpr uint32_t kTileBytes = kTile * kTK;  // one operand K step, 16 KB
+constexpr uint32_t kGemmThreads = 512;
+constexpr uint32_t kLoads = kTileBytes / 16 / kGemmThreads;  // 16-byte loads per thread per operand
+constexpr uint32_t kWaveM = 8, kWaveN = 4;    // 16x16 tiles per wave
+constexpr uint32_t kGroupM = 8;               // raster group height (tiles)
 
 void check_launch(const char* what) {
     hipError_t e = hipGetLastError();
@@ -80,188 +92,233 @@ __device__ __forceinline__ uint32_t redc_signed(int64_t t, uint32_t p, uint32_t 
     return uint32_t(r);
 }
 
-// LDS tile: rows of 64 bytes, 16-byte chunks XOR-swizzled by row/4 so the 16
-// rows one MFMA fragment reads land in distinct bank groups.
-__device__ __forceinline__ uint32_t lds_offset(uint32_t row, uint32_t chunk) {
-    return row * kBK + (((chunk ^ (row >> 2)) & 3u) << 4);
-}
-
 __device__ __forceinline__ int64_t combine4(const v4i& v) {
     return int64_t(v[0]) + int64_t(v[1]) * 256 + int64_t(v[2]) * 65536 + int64_t(v[3]) * 16777216;
 }
 
-// One 64-byte K step of the 128x128 block tile: wave (wm, wn) owns a 64x64
-// quadrant = 4x4 MFMA 16x16 tiles.  A rows = weight rows, B rows = coefficients.
-__device__ __forceinline__ void mfma_step(const int8_t* lds_a, const int8_t* lds_b, uint32_t lane,
-                                          uint32_t wm, uint32_t wn, v4i (&acc)[4][4]) {
-    v4i fa[4], fb[4];
-#pragma unroll
-    for (uint32_t mi = 0; mi < 4; ++mi) {
-        uint32_t row = wm * 64 + mi * 16 + (lane & 15);
-        fa[mi] = *reinterpret_cast<const v4i*>(lds_a + lds_offset(row, lane >> 4));
-    }
-#pragma unroll
-    for (uint32_t ni = 0; ni < 4; ++ni) {
-        uint32_t col = wn * 64 + ni * 16 + (lane & 15);
-        fb[ni] = *reinterpret_cast<const v4i*>(lds_b + lds_offset(col, lane >> 4));
-    }
-#pragma unroll
-    for (uint32_t mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (uint32_t ni = 0; ni < 4; ++ni)
-            acc[mi][ni] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
+// LDS operand tile: 256 rows of 64 bytes; the four 16-byte chunks of a row are
+// XOR-swizzled by row/4, so the 16 rows one MFMA fragment reads (one chunk
+// each) fall into 16 distinct 4-bank groups.
+__device__ __forceinline__ uint32_t swz(uint32_t row, uint32_t chunk) {
+    return row * kTK + (((chunk ^ (row >> 2)) & 3u) << 4);
 }
 
-// Weight tile 128 rows x 64 bytes: 2 x 16 bytes per thread.
-__device__ __forceinline__ void load_weights(const int8_t* A, uint32_t lda, uint32_t k0, uint32_t tid,
-                                             uint4 (&ra)[2]) {
+// Block -> (m tile, n tile).  Workgroups are dispatched to the 8 XCDs round
+// robin, so first regroup ids so that each XCD owns a contiguous range, then
+// walk that range in column-major groups of kGroupM tile rows: the blocks an
+// XCD runs together share A tiles (across a column) and B tiles (across a
+// group row) in that XCD's L2.
+__device__ __forceinline__ void tile_coords(uint32_t num_m, uint32_t num_n, uint32_t& tm, uint32_t& tn) {
+    const uint32_t bid = blockIdx.x, total = num_m * num_n;
+    const uint32_t per = total / 8;
+    const uint32_t logical = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
+    const uint32_t in_group = kGroupM * num_n;
+    const uint32_t group = logical / in_group, first_m = group * kGroupM;
+    const uint32_t height = num_m - first_m < kGroupM ? num_m - first_m : kGroupM;
+    const uint32_t local = logical % in_group;
+    tm = first_m + local % height;
+    tn = local / height;
+}
+
+// B operand loaders.  Each fills one 256 x 64-byte K step per call through
+// kLoads x 16 bytes per thread and writes it into the swizzled LDS tile.
+struct TiledB {  // pre-tiled signed bytes: [k steps][256 rows][64]
+    const int8_t* base;
+    __device__ __forceinline__ void load(uint32_t ks, uint32_t tid, v4u (&rb)[kLoads]) const {
+        const v4u* src = reinterpret_cast<const v4u*>(base + size_t(ks) * kTileBytes);
 #pragma unroll
-    for (uint32_t s = 0; s < 2; ++s) {
-        uint32_t idx = tid + kGemmThreads * s, row = idx >> 2, chunk = idx & 3;
-        ra[s] = *reinterpret_cast<const uint4*>(A + size_t(row) * lda + k0 + chunk * 16);
+        for (uint32_t s = 0; s < kLoads; ++s) rb[s] = src[tid + kGemmThreads * s];
+    }
+    __device__ __forceinline__ void store(int8_t* lds, uint32_t tid, const v4u (&rb)[kLoads]) const {
+#pragma unroll
+        for (uint32_t s = 0; s < kLoads; ++s) {
+            const uint32_t idx = tid + kGemmThreads * s;
+            *reinterpret_cast<v4u*>(lds + swz(idx >> 2, idx & 3)) = rb[s];
+        }
+    }
+};
+
+struct PlaneB {  // residues [k][n_pad] u32, transposed in registers to [coef][4 bytes x prime]
+    const uint32_t* plane;
+    uint32_t stride, n0, r;
+    // thread: primes 4*pg..4*pg+3 of the step, coefficients 2*cg, 2*cg+1
+    __device__ __forceinline__ void load(uint32_t ks, uint32_t tid, v4u (&rb)[kLoads]) const {
+        const uint32_t pg = tid >> 7, cg = tid & 127;
+        v2u v[4];
+#pragma unroll
+        for (uint32_t s = 0; s < 4; ++s) {
+            const uint32_t k = ks * 16 + 4 * pg + s;
+            v[s] = k < r ? *reinterpret_cast<const v2u*>(plane + size_t(k) * stride + n0 + 2 * cg)
+                         : v2u{0, 0};
+        }
+        rb[0] = v4u{v[0].x, v[1].x, v[2].x, v[3].x};  // coefficient 2*cg: primes 4pg..4pg+3
+        rb[1] = v4u{v[0].y, v[1].y, v[2].y, v[3].y};  // coefficient 2*cg+1
+    }
+    __device__ __forceinline__ void store(int8_t* lds, uint32_t tid, const v4u (&rb)[kLoads]) const {
+        const uint32_t pg = tid >> 7, cg = tid & 127;
+        const uint32_t f = 0x80808080u;
+        *reinterpret_cast<v4u*>(lds + swz(2 * cg, pg)) = rb[0] ^ f;
+        *reinterpret_cast<v4u*>(lds + swz(2 * cg + 1, pg)) = rb[1] ^ f;
+    }
+};
+
+// The K loop: A is a pre-tiled weight panel ([k steps][256][64]); LDS holds
+// two stages of (A, B) = 64 KB; one barrier per 64-byte K step.  Wave (wm, wn)
+// computes rows wm*128 .. +128 and coefficients wn*64 .. +64 of the tile.
+template <class BLoader>
+__device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel, const BLoader& bl,
+                                              uint32_t ksteps, int8_t* lds, v4i (&acc)[kWaveM][kWaveN]) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t wm = wave >> 2, wn = wave & 3;
+    v4u ra[kLoads], rb[kLoads];
+    auto load_a = [&](uint32_t ks) {
+        const v4u* src = reinterpret_cast<const v4u*>(a_panel + size_t(ks) * kTileBytes);
+#pragma unroll
+        for (uint32_t s = 0; s < kLoads; ++s) ra[s] = src[tid + kGemmThreads * s];
+    };
+    auto store = [&](uint32_t buf) {
+        int8_t* la = lds + buf * 2 * kTileBytes;
+#pragma unroll
+        for (uint32_t s = 0; s < kLoads; ++s) {
+            const uint32_t idx = tid + kGemmThreads * s;
+            *reinterpret_cast<v4u*>(la + swz(idx >> 2, idx & 3)) = ra[s];
+        }
+        bl.store(la + kTileBytes, tid, rb);
+    };
+
+    load_a(0);
+    bl.load(0, tid, rb);
+    store(0);
+    __syncthreads();
+    for (uint32_t ks = 0; ks < ksteps; ++ks) {
+        const uint32_t buf = ks & 1;
+        const bool more = ks + 1 < ksteps;
+        if (more) {
+            load_a(ks + 1);
+            bl.load(ks + 1, tid, rb);
+        }
+        const int8_t* la = lds + buf * 2 * kTileBytes;
+        const int8_t* lb = la + kTileBytes;
+        v4i fa[kWaveM], fb[kWaveN];
+#pragma unroll
+        for (uint32_t mi = 0; mi < kWaveM; ++mi)
+            fa[mi] = *reinterpret_cast<const v4i*>(la + swz(wm * 128 + mi * 16 + (lane & 15), lane >> 4));
+#pragma unroll
+        for (uint32_t ni = 0; ni < kWaveN; ++ni)
+            fb[ni] = *reinterpret_cast<const v4i*>(lb + swz(wn * 64 + ni * 16 + (lane & 15), lane >> 4));
+#pragma unroll
+        for (uint32_t mi = 0; mi < kWaveM; ++mi)
+#pragma unroll
+            for (uint32_t ni = 0; ni < kWaveN; ++ni)
+                acc[mi][ni] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
+        if (more) store(buf ^ 1);
+        __syncthreads();
     }
 }
 
-__device__ __forceinline__ void store_weights(int8_t* lds_a, uint32_t tid, const uint4 (&ra)[2]) {
+// ---------------------------------------------------------------------------
+// 0. operand packing: [N][L] u32 -> [2][n_pad/256][conv_k/64][256][64] bytes,
+//    each byte XOR 0x80 (u -> u-128 as int8); padding bytes are zero values.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict__ a,
+                                                     const uint32_t* __restrict__ b,
+                                                     uint4* __restrict__ packed, uint32_t n, uint32_t limbs,
+                                                     uint32_t ksteps, uint32_t ntiles) {
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t per_op = size_t(ntiles) * ksteps * kTile * 4;
+    if (idx >= 2 * per_op) return;
+    const uint32_t op = uint32_t(idx / per_op);
+    size_t rest = idx % per_op;
+    const uint32_t chunk = uint32_t(rest & 3), row = uint32_t((rest >> 2) & 255);
+    rest >>= 10;
+    const uint32_t ks = uint32_t(rest % ksteps), tn = uint32_t(rest / ksteps);
+    const uint32_t coef = tn * kTile + row, w0 = ks * 16 + chunk * 4;
+    const uint32_t* src = (op ? b : a) + size_t(coef) * limbs;
+    uint32_t v[4];
 #pragma unroll
-    for (uint32_t s = 0; s < 2; ++s) {
-        uint32_t idx = tid + kGemmThreads * s, row = idx >> 2, chunk = idx & 3;
-        *reinterpret_cast<uint4*>(lds_a + lds_offset(row, chunk)) = ra[s];
-    }
+    for (uint32_t c = 0; c < 4; ++c) v[c] = (coef < n && w0 + c < limbs) ? src[w0 + c] : 0u;
+    const uint32_t flip = 0x80808080u;
+    packed[idx] = make_uint4(v[0] ^ flip, v[1] ^ flip, v[2] ^ flip, v[3] ^ flip);
 }
 
 // ---------------------------------------------------------------------------
 // 1. conversion GEMM: G[4k+d][i] = sum_j digit_d(lambda*256^j mod p_k) * (byte_j(x_i)-128)
+//    then x_i*lambda mod p_k = REDC(sum_d 256^d G[4k+d][i] + off_k).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void conversion_gemm(const int8_t* __restrict__ weights,
-                                                       const uint32_t* __restrict__ a,
-                                                       const uint32_t* __restrict__ b,
+__global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict__ wtiles,
+                                                       const int8_t* __restrict__ packed,
                                                        const PrimeConst* __restrict__ consts,
-                                                       uint32_t* __restrict__ planes, uint32_t n,
-                                                       uint32_t limbs, uint32_t r, uint32_t conv_rows,
-                                                       uint32_t conv_k) {
-    __shared__ __attribute__((aligned(16))) int8_t lds_a[kBM * kBK];
-    __shared__ __attribute__((aligned(16))) int8_t lds_b[kBN * kBK];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t wm = wave >> 1, wn = wave & 1;
-    const uint32_t n0 = blockIdx.x * kBN, m0 = blockIdx.y * kBM, op = blockIdx.z;
-    const int8_t* A = weights + (size_t(op) * conv_rows + m0) * conv_k;
-    const uint32_t* src = op ? b : a;
-    const uint32_t kbytes = 4 * limbs;
+                                                       uint32_t* __restrict__ planes, uint32_t n, uint32_t n_pad,
+                                                       uint32_t r, uint32_t conv_rows, uint32_t ksteps) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t wm = wave >> 2, wn = wave & 3;
+    const uint32_t ntiles = n_pad / kTile, num_m = conv_rows / kTile;
+    uint32_t tm, tn;
+    tile_coords(num_m, 2 * ntiles, tm, tn);
+    const uint32_t op = tn / ntiles, tcol = tn % ntiles;
+    const int8_t* a_panel = wtiles + (size_t(op) * num_m + tm) * ksteps * kTileBytes;
+    TiledB bl{packed + (size_t(op) * ntiles + tcol) * ksteps * kTileBytes};
 
-    uint4 ra[2];
-    uint32_t rb[8];
-    auto load = [&](uint32_t k0) {
-        load_weights(A, conv_k, k0, tid, ra);
+    v4i acc[kWaveM][kWaveN];
 #pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) {
-            uint32_t idx = tid + kGemmThreads * s, coef = idx >> 4, dw = idx & 15;
-            uint32_t kb = k0 + 4 * dw, gc = n0 + coef;
-            uint32_t v = (gc < n && kb < kbytes) ? src[size_t(gc) * limbs + (kb >> 2)] : 0u;
-            rb[s] = v ^ 0x80808080u;
-        }
-    };
-    auto store = [&]() {
-        store_weights(lds_a, tid, ra);
+    for (uint32_t mi = 0; mi < kWaveM; ++mi)
 #pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) {
-            uint32_t idx = tid + kGemmThreads * s, coef = idx >> 4, dw = idx & 15;
-            *reinterpret_cast<uint32_t*>(lds_b + lds_offset(coef, dw >> 2) + (dw & 3) * 4) = rb[s];
-        }
-    };
-
-    v4i acc[4][4];
-#pragma unroll
-    for (uint32_t mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (uint32_t ni = 0; ni < 4; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
-
-    load(0);
-    for (uint32_t k0 = 0; k0 < conv_k; k0 += kBK) {
-        __syncthreads();
-        store();
-        __syncthreads();
-        if (k0 + kBK < conv_k) load(k0 + kBK);
-        mfma_step(lds_a, lds_b, lane, wm, wn, acc);
-    }
+        for (uint32_t ni = 0; ni < kWaveN; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
+    gemm_mainloop(a_panel, bl, ksteps, lds, acc);
 
     // Lane holds rows 4*(lane/16)+j (j = digit) of each 16-row tile: one prime.
+    const uint32_t m0 = tm * kTile + wm * 128, c0 = tcol * kTile + wn * 64;
 #pragma unroll
-    for (uint32_t mi = 0; mi < 4; ++mi) {
-        const uint32_t kp = (m0 + wm * 64 + mi * 16) / 4 + (lane >> 4);
+    for (uint32_t mi = 0; mi < kWaveM; ++mi) {
+        const uint32_t kp = (m0 + mi * 16) / 4 + (lane >> 4);
         if (kp >= r) continue;
         const PrimeConst pc = consts[kp];
         const int64_t off = op ? pc.off_b : pc.off_a;
-        uint32_t* out = planes + (size_t(op) * r + kp) * n;
+        uint32_t* out = planes + (size_t(op) * r + kp) * n_pad;
 #pragma unroll
-        for (uint32_t ni = 0; ni < 4; ++ni) {
-            const uint32_t coef = n0 + wn * 64 + ni * 16 + (lane & 15);
-            if (coef >= n) continue;
-            out[coef] = redc_signed(combine4(acc[mi][ni]) + off, pc.p, pc.pinv);
+        for (uint32_t ni = 0; ni < kWaveN; ++ni) {
+            const uint32_t coef = c0 + ni * 16 + (lane & 15);
+            if (coef < n) out[coef] = redc_signed(combine4(acc[mi][ni]) + off, pc.p, pc.pinv);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // 5. CRT GEMM: G[c][i] = sum_{k,d} w[c][4k+d] * (byte_d(v_{k,i}) - 128),
-//    words[j][i] = sum_t 256^t G[4j+t][i]
+//    words[j][i] = sum_t 256^t G[4j+t][i]   (int64, exact)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void crt_gemm(const int8_t* __restrict__ weights,
+__global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtiles,
                                                 const uint32_t* __restrict__ plane,
-                                                int64_t* __restrict__ words, uint32_t n, uint32_t r,
-                                                uint32_t crt_k, uint32_t crt_words) {
-    __shared__ __attribute__((aligned(16))) int8_t lds_a[kBM * kBK];
-    __shared__ __attribute__((aligned(16))) int8_t lds_b[kBN * kBK];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t wm = wave >> 1, wn = wave & 1;
-    const uint32_t n0 = blockIdx.x * kBN, m0 = blockIdx.y * kBM;
-    const int8_t* A = weights + size_t(m0) * crt_k;
+                                                int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
+                                                uint32_t r, uint32_t crt_rows, uint32_t ksteps,
+                                                uint32_t crt_words) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t wm = wave >> 2, wn = wave & 3;
+    const uint32_t ntiles = n_pad / kTile, num_m = crt_rows / kTile;
+    uint32_t tm, tn;
+    tile_coords(num_m, ntiles, tm, tn);
+    const int8_t* a_panel = wtiles + size_t(tm) * ksteps * kTileBytes;
+    PlaneB bl{plane, n_pad, tn * kTile, r};
 
-    uint4 ra[2];
-    uint32_t rb[8];
-    auto load = [&](uint32_t k0) {
-        load_weights(A, crt_k, k0, tid, ra);
+    v4i acc[kWaveM][kWaveN];
 #pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) {
-            uint32_t idx = tid + kGemmThreads * s, kk = idx >> 7, coef = idx & 127;
-            uint32_t k = k0 / 4 + kk, gc = n0 + coef;
-            uint32_t v = (k < r && gc < n) ? plane[size_t(k) * n + gc] : 0u;
-            rb[s] = v ^ 0x80808080u;
-        }
-    };
-    auto store = [&]() {
-        store_weights(lds_a, tid, ra);
+    for (uint32_t mi = 0; mi < kWaveM; ++mi)
 #pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) {
-            uint32_t idx = tid + kGemmThreads * s, kk = idx >> 7, coef = idx & 127;
-            *reinterpret_cast<uint32_t*>(lds_b + lds_offset(coef, kk >> 2) + (kk & 3) * 4) = rb[s];
-        }
-    };
+        for (uint32_t ni = 0; ni < kWaveN; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
+    gemm_mainloop(a_panel, bl, ksteps, lds, acc);
 
-    v4i acc[4][4];
+    const uint32_t m0 = tm * kTile + wm * 128, c0 = tn * kTile + wn * 64;
 #pragma unroll
-    for (uint32_t mi = 0; mi < 4; ++mi)
-#pragma unroll
-        for (uint32_t ni = 0; ni < 4; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
-
-    load(0);
-    for (uint32_t k0 = 0; k0 < crt_k; k0 += kBK) {
-        __syncthreads();
-        store();
-        __syncthreads();
-        if (k0 + kBK < crt_k) load(k0 + kBK);
-        mfma_step(lds_a, lds_b, lane, wm, wn, acc);
-    }
-
-#pragma unroll
-    for (uint32_t mi = 0; mi < 4; ++mi) {
-        const uint32_t word = (m0 + wm * 64 + mi * 16) / 4 + (lane >> 4);
+    for (uint32_t mi = 0; mi < kWaveM; ++mi) {
+        const uint32_t word = (m0 + mi * 16) / 4 + (lane >> 4);
         if (word >= crt_words) continue;
+        int64_t* out = words + size_t(word) * n_pad;
 #pragma unroll
-        for (uint32_t ni = 0; ni < 4; ++ni) {
-            const uint32_t coef = n0 + wn * 64 + ni * 16 + (lane & 15);
-            if (coef >= n) continue;
-            words[size_t(word) * n + coef] = combine4(acc[mi][ni]);
+        for (uint32_t ni = 0; ni < kWaveN; ++ni) {
+            const uint32_t coef = c0 + ni * 16 + (lane & 15);
+            if (coef < n) out[coef] = combine4(acc[mi][ni]);
         }
     }
 }
@@ -270,16 +327,17 @@ __global__ __launch_bounds__(256) void crt_gemm(const int8_t* __restrict__ weigh
 // 2-4. NTT.  Flat index i = row*C + col.  Forward: Cooley-Tukey with the
 // negacyclic twist merged into psi^bitrev twiddles, m = 1, 2, .., n2/2 groups
 // at flat distance N/(2m) down to n1.  Inverse: Gentleman-Sande, mirror order.
+// Plane rows have stride ps (= n_pad).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void ntt_columns_forward(uint32_t* __restrict__ planes,
                                                            const PrimeConst* __restrict__ consts,
-                                                           const uint2* __restrict__ tw, uint32_t n,
+                                                           const uint2* __restrict__ tw, uint32_t ps,
                                                            uint32_t n2, uint32_t C, uint32_t rows,
                                                            uint32_t cw, uint32_t r) {
     extern __shared__ uint32_t tile[];  // [rows][cw]
     const uint32_t col0 = blockIdx.x * cw, k = blockIdx.y, op = blockIdx.z;
     const uint32_t p = consts[k].p;
-    uint32_t* data = planes + (size_t(op) * r + k) * n;
+    uint32_t* data = planes + (size_t(op) * r + k) * ps;
     const uint2* w = tw + size_t(k) * n2;
     const uint32_t total = rows * cw;
     for (uint32_t t = threadIdx.x; t < total; t += blockDim.x)
@@ -304,16 +362,16 @@ __global__ __launch_bounds__(256) void ntt_columns_forward(uint32_t* __restrict_
 __global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ planes,
                                                         const PrimeConst* __restrict__ consts,
                                                         const uint2* __restrict__ twf,
-                                                        const uint2* __restrict__ twi, uint32_t n,
+                                                        const uint2* __restrict__ twi, uint32_t ps,
                                                         uint32_t n1, uint32_t n2, uint32_t C,
                                                         uint32_t rows, uint32_t r) {
-    extern __shared__ uint32_t lds[];  // A[C], B[C]
-    uint32_t* A = lds;
-    uint32_t* B = lds + C;
+    extern __shared__ uint32_t lds32[];  // A[C], B[C]
+    uint32_t* A = lds32;
+    uint32_t* B = lds32 + C;
     const uint32_t row = blockIdx.x, k = blockIdx.y;
     const uint32_t p = consts[k].p, pinv = consts[k].pinv;
-    uint32_t* pa = planes + size_t(k) * n + size_t(row) * C;
-    const uint32_t* pb = planes + (size_t(r) + k) * n + size_t(row) * C;
+    uint32_t* pa = planes + size_t(k) * ps + size_t(row) * C;
+    const uint32_t* pb = planes + (size_t(r) + k) * ps + size_t(row) * C;
     const uint2* wf = twf + size_t(k) * n2;
     const uint2* wi = twi + size_t(k) * n2;
     for (uint32_t t = threadIdx.x; t < C; t += blockDim.x) {
@@ -382,13 +440,13 @@ __global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ p
 
 __global__ __launch_bounds__(256) void ntt_columns_inverse(uint32_t* __restrict__ plane,
                                                            const PrimeConst* __restrict__ consts,
-                                                           const uint2* __restrict__ tw, uint32_t n,
+                                                           const uint2* __restrict__ tw, uint32_t ps,
                                                            uint32_t n2, uint32_t C, uint32_t rows,
                                                            uint32_t cw) {
     extern __shared__ uint32_t tile[];  // [rows][cw]
     const uint32_t col0 = blockIdx.x * cw, k = blockIdx.y;
     const uint32_t p = consts[k].p;
-    uint32_t* data = plane + size_t(k) * n;
+    uint32_t* data = plane + size_t(k) * ps;
     const uint2* w = tw + size_t(k) * n2;
     const uint32_t total = rows * cw;
     for (uint32_t t = threadIdx.x; t < total; t += blockDim.x)
@@ -410,10 +468,10 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse(uint32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// 6. reconstruction: one lane per coefficient, words in [j][N] layout.
+// 6. reconstruction: one lane per coefficient, words in [j][n_pad] layout.
 // ---------------------------------------------------------------------------
 struct ReconstructArgs {
-    uint32_t n, limbs, t_words;
+    uint32_t n, n_pad, limbs, t_words;
     uint32_t s_shift, k_shift;
     uint64_t mu;
 };
@@ -427,7 +485,7 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
                                                   ReconstructArgs args) {
     __shared__ uint32_t tile[64][65];
     const uint32_t lane = threadIdx.x, i0 = blockIdx.x * 64, i = i0 + lane;
-    const uint32_t n = args.n, L = args.limbs, tw = args.t_words;
+    const uint32_t n = args.n, ps = args.n_pad, L = args.limbs, tw = args.t_words;
     const bool valid = i < n;
     const uint32_t ic = valid ? i : 0;  // clamp reads of idle lanes
 
@@ -436,7 +494,7 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     {
         int64_t carry = 0;
         for (uint32_t j = 0; j < 4; ++j) {
-            int64_t x = (j < 3 ? words[size_t(L + 1 + j) * n + ic] : 0) + int64_t(corr_e[j]) + carry;
+            int64_t x = (j < 3 ? words[size_t(L + 1 + j) * ps + ic] : 0) + int64_t(corr_e[j]) + carry;
             ew[j] = uint32_t(uint64_t(x));
             carry = x >> 32;
         }
@@ -449,11 +507,11 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     {
         int64_t carry = 0;
         for (uint32_t j = 0; j < tw; ++j) {
-            int64_t x = (j <= L ? words[size_t(j) * n + ic] : 0) + int64_t(corr_t[j]) + carry;
+            int64_t x = (j <= L ? words[size_t(j) * ps + ic] : 0) + int64_t(corr_t[j]) + carry;
             if (j < L) x += int64_t(kq) * int64_t(neg_p[j]);
             const uint32_t tj = uint32_t(uint64_t(x));
             carry = x >> 32;
-            if (valid) tsc[size_t(j) * n + ic] = tj;
+            if (valid) tsc[size_t(j) * ps + ic] = tj;
             if (j >= sw && j < sw + 3) top[j - sw] = tj;
         }
     }
@@ -473,7 +531,7 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
             const unsigned __int128 pr = (unsigned __int128)qe * qj + pc;
             const uint32_t x = uint32_t(pr);
             pc = uint64_t(pr >> 32);
-            const uint32_t tj = tsc[size_t(j) * n + ic];
+            const uint32_t tj = tsc[size_t(j) * ps + ic];
             const uint64_t d = uint64_t(tj) - x - bo;
             const uint32_t rj = uint32_t(d);
             bo = uint32_t(d >> 63);
@@ -514,13 +572,26 @@ uint32_t column_width(const Shape& s) {
     return cw;
 }
 
+constexpr size_t kGemmLds = 4 * size_t(kTileBytes);  // two stages of (A, B)
+
 }  // namespace
 
-void launch_conversion(const Shape& s, const DeviceTables& t, const uint32_t* a, const uint32_t* b,
-                       uint32_t* planes, hipStream_t stream) {
-    dim3 grid((s.n + kBN - 1) / kBN, s.conv_rows / kBM, 2);
-    hipLaunchKernelGGL(conversion_gemm, grid, dim3(kGemmThreads), 0, stream, t.wconv, a, b, t.consts, planes,
-                       s.n, s.limbs, s.r, s.conv_rows, s.conv_k);
+size_t packed_operand_bytes(const Shape& s) { return size_t(2) * s.n_pad * s.conv_k; }
+
+void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, hipStream_t stream) {
+    const uint32_t ksteps = s.conv_k / kTK, ntiles = s.n_pad / kTile;
+    const size_t chunks = size_t(2) * ntiles * ksteps * kTile * 4;
+    dim3 grid(uint32_t((chunks + 255) / 256));
+    hipLaunchKernelGGL(pack_operands, grid, dim3(256), 0, stream, a, b, reinterpret_cast<uint4*>(packed), s.n,
+                       s.limbs, ksteps, ntiles);
+    check_launch("pack_operands");
+}
+
+void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
+                       hipStream_t stream) {
+    const uint32_t blocks = (s.conv_rows / kTile) * 2 * (s.n_pad / kTile);
+    hipLaunchKernelGGL(conversion_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wconv, packed,
+                       t.consts, planes, s.n, s.n_pad, s.r, s.conv_rows, s.conv_k / kTK);
     check_launch("conversion_gemm");
 }
 
@@ -529,14 +600,14 @@ void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t*
     const uint32_t cw = column_width(s);
     dim3 grid(s.row_width / cw, s.r, 2);
     hipLaunchKernelGGL(ntt_columns_forward, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes, t.consts,
-                       t.tw_fwd, s.n, s.n2, s.row_width, s.rows, cw, s.r);
+                       t.tw_fwd, s.n_pad, s.n2, s.row_width, s.rows, cw, s.r);
     check_launch("ntt_columns_forward");
 }
 
 void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream) {
     dim3 grid(s.rows, s.r);
     hipLaunchKernelGGL(ntt_rows_product, grid, dim3(256), size_t(2) * s.row_width * 4, stream, planes, t.consts,
-                       t.tw_fwd, t.tw_inv, s.n, s.n1, s.n2, s.row_width, s.rows, s.r);
+                       t.tw_fwd, t.tw_inv, s.n_pad, s.n1, s.n2, s.row_width, s.rows, s.r);
     check_launch("ntt_rows_product");
 }
 
@@ -545,21 +616,21 @@ void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t*
     const uint32_t cw = column_width(s);
     dim3 grid(s.row_width / cw, s.r);
     hipLaunchKernelGGL(ntt_columns_inverse, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes, t.consts,
-                       t.tw_inv, s.n, s.n2, s.row_width, s.rows, cw);
+                       t.tw_inv, s.n_pad, s.n2, s.row_width, s.rows, cw);
     check_launch("ntt_columns_inverse");
 }
 
 void launch_crt(const Shape& s, const DeviceTables& t, const uint32_t* product_plane, int64_t* words,
                 hipStream_t stream) {
-    dim3 grid((s.n + kBN - 1) / kBN, s.crt_rows / kBM);
-    hipLaunchKernelGGL(crt_gemm, grid, dim3(kGemmThreads), 0, stream, t.wcrt, product_plane, words, s.n, s.r,
-                       s.crt_k, s.crt_words);
+    const uint32_t blocks = (s.crt_rows / kTile) * (s.n_pad / kTile);
+    hipLaunchKernelGGL(crt_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, product_plane, words,
+                       s.n, s.n_pad, s.r, s.crt_rows, s.crt_k / kTK, s.crt_words);
     check_launch("crt_gemm");
 }
 
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
                         uint32_t* c, hipStream_t stream) {
-    ReconstructArgs args{s.n, s.limbs, s.t_words, s.s_shift, s.k_shift, s.mu};
+    ReconstructArgs args{s.n, s.n_pad, s.limbs, s.t_words, s.s_shift, s.k_shift, s.mu};
     dim3 grid((s.n + 63) / 64);
     hipLaunchKernelGGL(reconstruct, grid, dim3(64), 0, stream, words, t_scratch, c, t.q, t.neg_p, t.corr_t,
                        t.corr_e, args);
@@ -571,6 +642,19 @@ void launch_zero(uint32_t* c, size_t words, hipStream_t stream) {
     dim3 grid(uint32_t((words + 255) / 256));
     hipLaunchKernelGGL(fill_zero, grid, dim3(256), 0, stream, c, words);
     check_launch("fill_zero");
+}
+
+void configure_kernels() {
+    // The GEMMs use 64 KB of dynamic LDS (two stages of A and B tiles).
+    static bool done = false;
+    if (done) return;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conversion_gemm),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(kGemmLds));
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    done = true;
 }
 
 }  // namespace pm

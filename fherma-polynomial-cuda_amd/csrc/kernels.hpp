@@ -12,26 +12,32 @@ namespace pm {
 // Device views of the plan's tables.
 struct DeviceTables {
     const PrimeConst* consts = nullptr;   // [r]
-    const int8_t* wconv = nullptr;        // [2][conv_rows][conv_k]
+    const int8_t* wconv = nullptr;        // [2] x tile_256x64([conv_rows][conv_k])
     const uint2* tw_fwd = nullptr;        // [r][n2]
     const uint2* tw_inv = nullptr;        // [r][n2]
-    const int8_t* wcrt = nullptr;         // [crt_rows][crt_k]
+    const int8_t* wcrt = nullptr;         // tile_256x64([crt_rows][crt_k])
     const uint32_t* q = nullptr;          // [L]
     const uint32_t* neg_p = nullptr;      // [L]
     const uint32_t* corr_t = nullptr;     // [t_words]
     const uint32_t* corr_e = nullptr;     // [4]
 };
 
-// Working buffers.
+// Working buffers (row stride n_pad).
 struct DeviceWork {
-    uint32_t* planes = nullptr;  // [2][r][N] residues; plane 0 receives v_k
-    int64_t* words = nullptr;    // [crt_words][N] CRT word sums
-    uint32_t* t_words = nullptr; // [t_words][N] T scratch (aliases planes)
+    int8_t* packed = nullptr;    // [2][n_pad/256][conv_k/64][256][64] signed operand bytes
+    uint32_t* planes = nullptr;  // [2][r][n_pad] residues; plane 0 receives v_k
+    int64_t* words = nullptr;    // [crt_words][n_pad] CRT word sums
+    uint32_t* t_words = nullptr; // [t_words][n_pad] T scratch (aliases the second plane)
 };
 
+// One-time kernel attributes (dynamic LDS limits); call before any launch.
+void configure_kernels();
+size_t packed_operand_bytes(const Shape& s);
+
 // Stage launches, all on `stream`.
-void launch_conversion(const Shape& s, const DeviceTables& t, const uint32_t* a, const uint32_t* b,
-                       uint32_t* planes, hipStream_t stream);
+void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, hipStream_t stream);
+void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
+                       hipStream_t stream);
 void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
 void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
 void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);

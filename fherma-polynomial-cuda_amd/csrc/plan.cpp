@@ -109,6 +109,19 @@ bool is_prime_u32(uint32_t p) {
     return true;
 }
 
+std::vector<int8_t> tile_256x64(const std::vector<int8_t>& src, uint32_t rows, uint32_t cols) {
+    if (rows % 256 || cols % 64 || src.size() != size_t(rows) * cols)
+        throw std::logic_error("plan: tiling shape mismatch");
+    std::vector<int8_t> out(src.size());
+    const uint32_t ksteps = cols / 64;
+    for (uint32_t r = 0; r < rows; ++r)
+        for (uint32_t ks = 0; ks < ksteps; ++ks) {
+            const size_t dst = ((size_t(r / 256) * ksteps + ks) * 256 + (r % 256)) * 64;
+            std::memcpy(out.data() + dst, src.data() + size_t(r) * cols + size_t(ks) * 64, 64);
+        }
+    return out;
+}
+
 bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes) {
     primes->clear();
     Words product{1};
@@ -164,11 +177,12 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
     s.reference_walk = (n1 == 1);
     s.r = uint32_t(plan.primes.size());
     if (s.r >= 16384) throw std::runtime_error("modulus too wide: more than 16383 NTT fields needed");
-    s.r_pad = (s.r + 31) / 32 * 32;
+    s.n_pad = (n + 255) / 256 * 256;
+    s.r_pad = (s.r + 63) / 64 * 64;
     s.conv_k = (4 * limbs + 63) / 64 * 64;
     s.conv_rows = 4 * s.r_pad;
     s.crt_words = limbs + 4;
-    s.crt_words_pad = (s.crt_words + 31) / 32 * 32;
+    s.crt_words_pad = (s.crt_words + 63) / 64 * 64;
     s.crt_rows = 4 * s.crt_words_pad;
     s.crt_k = 4 * s.r_pad;
     s.t_words = limbs + 2;

@@ -26,7 +26,8 @@ struct Shape {
     // polynomials mod (X^n1 - zeta).  n1 = 1 is the plain negacyclic NTT.
     uint32_t n1 = 1, n2 = 1, log_n2 = 0;
     uint32_t r = 0;          // primes
-    uint32_t r_pad = 0;      // primes padded to a multiple of 32
+    uint32_t n_pad = 0;      // N padded to the 256-coefficient GEMM tile: row stride of planes/words
+    uint32_t r_pad = 0;      // primes padded to a multiple of 64 (256 GEMM rows)
     uint32_t conv_k = 0;     // conversion GEMM K: 4L bytes padded to 64
     uint32_t conv_rows = 0;  // conversion GEMM rows: 4 digits per (padded) prime
     uint32_t crt_words = 0;  // CRT GEMM output words: L+1 sum words + 3 quotient words
@@ -73,5 +74,10 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
 bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes);
 
 bool is_prime_u32(uint32_t p);
+
+// GEMM operand tiling: a row-major [rows][cols] byte matrix (rows a multiple
+// of 256, cols a multiple of 64) as [rows/256][cols/64][256][64], so that every
+// (256-row tile, 64-byte K step) block is one contiguous 16 KB chunk.
+std::vector<int8_t> tile_256x64(const std::vector<int8_t>& src, uint32_t rows, uint32_t cols);
 
 }  // namespace pm

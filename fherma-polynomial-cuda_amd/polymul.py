@@ -46,7 +46,7 @@ SHAPE_FIELDS = ("n", "width", "limbs", "n1", "n2", "log_n2", "r", "r_pad", "conv
                 "crt_words", "crt_words_pad", "crt_rows", "crt_k", "t_words", "row_width", "rows",
                 "qbits", "s_shift", "k_shift", "mu_lo", "mu_hi", "trivial", "reference_walk")
 
-STAGE_TIME_NAMES = ("conversion_gemm", "ntt_columns_forward", "ntt_rows_product",
+STAGE_TIME_NAMES = ("pack_operands", "conversion_gemm", "ntt_columns_forward", "ntt_rows_product",
                     "ntt_columns_inverse", "crt_gemm", "reconstruct", "total")
 
 
