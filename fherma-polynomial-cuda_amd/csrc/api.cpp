@@ -121,12 +121,13 @@ struct pm_plan {
         tables.corr_t = upload(host.corr_t);
         tables.corr_e = upload(host.corr_e);
         work.packed = alloc<int8_t>(pm::packed_operand_bytes(s));
-        // planes: [2][r][n_pad]; after the row kernel the second plane is free
-        // and holds the reconstruction's T scratch ([L+2][n_pad]).
-        size_t plane = size_t(s.r) * s.n_pad;
-        size_t scratch = size_t(s.t_words) * s.n_pad;
-        work.planes = alloc<uint32_t>(plane + (plane > scratch ? plane : scratch));
-        work.t_words = work.planes + plane;
+        // planes: [2][r_pad][n_pad].  After the row stages the b half is dead and
+        // receives the CRT operand tiles of v_k (exactly r_pad*n_pad words);
+        // after the CRT GEMM the a half is dead and holds T ([L+2][n_pad]).
+        const size_t half = size_t(s.r_pad > s.t_words ? s.r_pad : s.t_words) * s.n_pad;
+        work.planes = alloc<uint32_t>(2 * half);
+        work.rtiles = reinterpret_cast<int8_t*>(work.planes + half);
+        work.t_words = work.planes;
         work.words = alloc<int64_t>(size_t(s.crt_words) * s.n_pad);
     }
 
@@ -150,10 +151,10 @@ struct pm_plan {
         mark(3);
         pm::launch_ntt_rows_product(s, tables, work.planes, st);
         mark(4);
-        pm::launch_ntt_inverse_columns(s, tables, work.planes, st);
+        pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
         mark(5);
         if (last == PM_STAGE_PRODUCT) return;
-        pm::launch_crt(s, tables, work.planes, work.words, st);
+        pm::launch_crt(s, tables, work.rtiles, work.words, st);
         mark(6);
         if (last == PM_STAGE_WORDS) return;
         pm::launch_reconstruct(s, tables, work.words, work.t_words, c, st);
@@ -356,17 +357,32 @@ pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, in
         plan->run(plan->io_a, plan->io_b, plan->io_c, stage, st);
         if (stage == PM_STAGE_OUTPUT) {
             hip_check(hipMemcpyAsync(out, plan->io_c, need, hipMemcpyDeviceToHost, st), "copy stage");
+        } else if (stage == PM_STAGE_PRODUCT) {
+            // v_k lives in the CRT operand tiles: [n_pad/256][r_pad/16][256][16 x u32], XOR 0x80808080
+            const size_t tile_words = size_t(s.r_pad) * s.n_pad;
+            std::vector<uint32_t> tiles(tile_words);
+            hip_check(hipMemcpyAsync(tiles.data(), plan->work.rtiles, tile_words * 4, hipMemcpyDeviceToHost, st),
+                      "copy stage");
+            hip_check(hipStreamSynchronize(st), "debug stage");
+            uint32_t* v = static_cast<uint32_t*>(out);
+            const size_t ksteps = s.r_pad / 16;
+            for (size_t k = 0; k < s.r; ++k)
+                for (size_t i = 0; i < s.n; ++i)
+                    v[k * s.n + i] = tiles[(((i >> 8) * ksteps + k / 16) * 256 + (i & 255)) * 16 + k % 16] ^ 0x80808080u;
         } else {
             // device rows have stride n_pad; the tap returns rows of exactly N
             const size_t elem = stage == PM_STAGE_WORDS ? 8 : 4;
-            const size_t rows = stage == PM_STAGE_WORDS ? s.crt_words
-                                : stage == PM_STAGE_RESIDUES ? 2 * size_t(s.r)
-                                                             : size_t(s.r);
-            const void* src = stage == PM_STAGE_WORDS ? static_cast<const void*>(plan->work.words)
-                                                      : static_cast<const void*>(plan->work.planes);
-            hip_check(hipMemcpy2DAsync(out, s.n * elem, src, size_t(s.n_pad) * elem, s.n * elem, rows,
-                                       hipMemcpyDeviceToHost, st),
-                      "copy stage");
+            if (stage == PM_STAGE_WORDS) {
+                hip_check(hipMemcpy2DAsync(out, s.n * elem, plan->work.words, size_t(s.n_pad) * elem, s.n * elem,
+                                           s.crt_words, hipMemcpyDeviceToHost, st),
+                          "copy stage");
+            } else {  // residues of a then b; the b plane starts r_pad rows in
+                for (size_t op = 0; op < 2; ++op)
+                    hip_check(hipMemcpy2DAsync(static_cast<uint8_t*>(out) + op * s.r * s.n * elem, s.n * elem,
+                                               plan->work.planes + op * s.r_pad * size_t(s.n_pad),
+                                               size_t(s.n_pad) * elem, s.n * elem, s.r, hipMemcpyDeviceToHost, st),
+                              "copy stage");
+            }
         }
         hip_check(hipStreamSynchronize(st), "debug stage");
         *bytes = need;

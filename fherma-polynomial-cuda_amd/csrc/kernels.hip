@@ -138,30 +138,6 @@ struct TiledB {  // pre-tiled signed bytes: [k steps][256 rows][64]
     }
 };
 
-struct PlaneB {  // residues [k][n_pad] u32, transposed in registers to [coef][4 bytes x prime]
-    const uint32_t* plane;
-    uint32_t stride, n0, r;
-    // thread: primes 4*pg..4*pg+3 of the step, coefficients 2*cg, 2*cg+1
-    __device__ __forceinline__ void load(uint32_t ks, uint32_t tid, v4u (&rb)[kLoads]) const {
-        const uint32_t pg = tid >> 7, cg = tid & 127;
-        v2u v[4];
-#pragma unroll
-        for (uint32_t s = 0; s < 4; ++s) {
-            const uint32_t k = ks * 16 + 4 * pg + s;
-            v[s] = k < r ? *reinterpret_cast<const v2u*>(plane + size_t(k) * stride + n0 + 2 * cg)
-                         : v2u{0, 0};
-        }
-        rb[0] = v4u{v[0].x, v[1].x, v[2].x, v[3].x};  // coefficient 2*cg: primes 4pg..4pg+3
-        rb[1] = v4u{v[0].y, v[1].y, v[2].y, v[3].y};  // coefficient 2*cg+1
-    }
-    __device__ __forceinline__ void store(int8_t* lds, uint32_t tid, const v4u (&rb)[kLoads]) const {
-        const uint32_t pg = tid >> 7, cg = tid & 127;
-        const uint32_t f = 0x80808080u;
-        *reinterpret_cast<v4u*>(lds + swz(2 * cg, pg)) = rb[0] ^ f;
-        *reinterpret_cast<v4u*>(lds + swz(2 * cg + 1, pg)) = rb[1] ^ f;
-    }
-};
-
 // The K loop: A is a pre-tiled weight panel ([k steps][256][64]); LDS holds
 // two stages of (A, B) = 64 KB; one barrier per 64-byte K step.  Wave (wm, wn)
 // computes rows wm*128 .. +128 and coefficients wn*64 .. +64 of the tile.
@@ -249,7 +225,8 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
                                                        const int8_t* __restrict__ packed,
                                                        const PrimeConst* __restrict__ consts,
                                                        uint32_t* __restrict__ planes, uint32_t n, uint32_t n_pad,
-                                                       uint32_t r, uint32_t conv_rows, uint32_t ksteps) {
+                                                       uint32_t r, uint32_t r_pad, uint32_t conv_rows,
+                                                       uint32_t ksteps) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave >> 2, wn = wave & 3;
@@ -275,7 +252,7 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
         if (kp >= r) continue;
         const PrimeConst pc = consts[kp];
         const int64_t off = op ? pc.off_b : pc.off_a;
-        uint32_t* out = planes + (size_t(op) * r + kp) * n_pad;
+        uint32_t* out = planes + (size_t(op) * r_pad + kp) * n_pad;
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni) {
             const uint32_t coef = c0 + ni * 16 + (lane & 15);
@@ -289,10 +266,9 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
 //    words[j][i] = sum_t 256^t G[4j+t][i]   (int64, exact)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtiles,
-                                                const uint32_t* __restrict__ plane,
+                                                const int8_t* __restrict__ rtiles,
                                                 int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
-                                                uint32_t r, uint32_t crt_rows, uint32_t ksteps,
-                                                uint32_t crt_words) {
+                                                uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave >> 2, wn = wave & 3;
@@ -300,7 +276,7 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
     uint32_t tm, tn;
     tile_coords(num_m, ntiles, tm, tn);
     const int8_t* a_panel = wtiles + size_t(tm) * ksteps * kTileBytes;
-    PlaneB bl{plane, n_pad, tn * kTile, r};
+    TiledB bl{rtiles + size_t(tn) * ksteps * kTileBytes};
 
     v4i acc[kWaveM][kWaveN];
 #pragma unroll
@@ -468,6 +444,234 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse(uint32_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Register-resident NTT kernels for the usual shapes (rows <= 64, C = 2048).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void ct_butterfly(uint32_t& a, uint32_t& b, uint2 w, uint32_t p) {
+    const uint32_t v = mul_shoup(b, w.x, w.y, p);
+    b = sub_mod(a, v, p);
+    a = add_mod(a, v, p);
+}
+
+__device__ __forceinline__ void gs_butterfly(uint32_t& a, uint32_t& b, uint2 w, uint32_t p) {
+    const uint32_t u = a, v = b;
+    a = add_mod(u, v, p);
+    b = mul_shoup(sub_mod(u, v, p), w.x, w.y, p);
+}
+
+// Forward row-spanning stages: one thread per (operand, prime, column) holding
+// the column's ROWS values; the stage twiddles are uniform over the block.
+template <int ROWS>
+__global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restrict__ planes,
+                                                               const PrimeConst* __restrict__ consts,
+                                                               const uint2* __restrict__ tw, uint32_t ps,
+                                                               uint32_t n2, uint32_t C, uint32_t r) {
+    const uint32_t col = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, op = blockIdx.z;
+    const uint32_t p = consts[k].p;
+    uint32_t* data = planes + (size_t(op) * r + k) * ps + col;
+    const uint2* w = tw + size_t(k) * n2;
+    uint32_t x[ROWS];
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) x[j] = data[size_t(j) * C];
+#pragma unroll
+    for (int m = 1, half = ROWS / 2; m < ROWS; m *= 2, half /= 2)
+#pragma unroll
+        for (int g = 0; g < m; ++g) {
+            const uint2 t = w[m + g];
+#pragma unroll
+            for (int jr = 0; jr < half; ++jr) ct_butterfly(x[2 * half * g + jr], x[2 * half * g + jr + half], t, p);
+        }
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) data[size_t(j) * C] = x[j];
+}
+
+// Inverse row-spanning stages for 16 primes x 16 columns per block, then the
+// residues v_{k,i} are written, XOR 0x80808080, into the CRT GEMM's operand
+// tiles [n_pad/256][r_pad/16][256 coefficients][16 primes x 4 bytes] through
+// an LDS transpose, so each coefficient's 64-byte K chunk is one store.
+// ROWS = 1 is a pure transpose (no stages).
+template <int ROWS>
+__global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t* __restrict__ plane,
+                                                                 const PrimeConst* __restrict__ consts,
+                                                                 const uint2* __restrict__ tw, uint32_t ps,
+                                                                 uint32_t n, uint32_t n2, uint32_t C, uint32_t r,
+                                                                 uint32_t ksteps, int8_t* __restrict__ tiles) {
+    __shared__ uint32_t lds[ROWS][16][17];
+    const uint32_t kk = threadIdx.x >> 4, cc = threadIdx.x & 15;
+    const uint32_t c0 = blockIdx.x * 16, kg = blockIdx.y, k = kg * 16 + kk;
+    uint32_t x[ROWS];
+    if (k < r) {
+        const uint32_t p = consts[k].p;
+        const uint32_t* data = plane + size_t(k) * ps + c0 + cc;
+        const uint2* w = tw + size_t(k) * n2;
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) x[j] = data[size_t(j) * C];
+#pragma unroll
+        for (int m = ROWS / 2, half = 1; m >= 1; m /= 2, half *= 2)
+#pragma unroll
+            for (int g = 0; g < m; ++g) {
+                const uint2 t = w[m + g];
+#pragma unroll
+                for (int jr = 0; jr < half; ++jr) gs_butterfly(x[2 * half * g + jr], x[2 * half * g + jr + half], t, p);
+            }
+    } else {
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) x[j] = 0;
+    }
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) lds[j][cc][kk] = x[j] ^ 0x80808080u;
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < uint32_t(ROWS) * 64; idx += 256) {
+        const uint32_t q = idx & 3, c2 = (idx >> 2) & 15, j = idx >> 6;
+        const uint32_t i = j * C + c0 + c2;
+        if (i < n) {
+            const v4u v{lds[j][c2][4 * q], lds[j][c2][4 * q + 1], lds[j][c2][4 * q + 2], lds[j][c2][4 * q + 3]};
+            *reinterpret_cast<v4u*>(tiles + ((size_t(i >> 8) * ksteps + kg) * 256 + (i & 255)) * 64 + q * 16) = v;
+        }
+    }
+}
+
+// Row stages for C = 2048: one block per (row, prime), 256 threads x 8 values
+// per operand.  The 11 butterfly levels inside a row are done three at a time
+// in registers, with an LDS exchange between groups:
+//   layout A: jj = t + 256e      distances 1024, 512, 256
+//   layout B: jj = 256(t/32) + (t%32) + 32e   distances 128, 64, 32
+//   layout C: jj = 32(t/4) + (t%4) + 4e       distances 16, 8, 4
+//   layout D: jj = 8t + e        distances 2, 1, and the base case (n1 <= 8)
+// Forward stops at distance n1, the base case multiplies each n1-block mod
+// (X^n1 - zeta), and the inverse climbs back from n1 to 1024.
+constexpr uint32_t kRowC = 2048;
+
+__device__ __forceinline__ uint32_t layout_index(int layout, uint32_t t, int e) {
+    switch (layout) {
+        case 0: return t + 256u * e;
+        case 1: return 256u * (t >> 5) + (t & 31u) + 32u * e;
+        case 2: return 32u * (t >> 2) + (t & 3u) + 4u * e;
+        default: return 8u * t + e;
+    }
+}
+
+// forward levels of one layout: distances stride*2^(levels-1) .. stride (those >= n1)
+__device__ __forceinline__ void row_forward_group(uint32_t (&x)[8], int layout, int levels, uint32_t t,
+                                                  uint32_t stride, uint32_t n1, uint32_t base_index,
+                                                  const uint2* w, uint32_t p) {
+#pragma unroll
+    for (int b = levels - 1; b >= 0; --b) {
+        const uint32_t d = stride << b;
+        if (d < n1) continue;
+        const uint32_t shift = 31 - __clz(2 * d);  // log2(2d)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (e & (1 << b)) continue;
+            const uint32_t jj = layout_index(layout, t, e);
+            const uint2 tw = w[(base_index >> shift) + (jj >> shift)];
+            ct_butterfly(x[e], x[e + (1 << b)], tw, p);
+        }
+    }
+}
+
+__device__ __forceinline__ void row_inverse_group(uint32_t (&x)[8], int layout, int levels, uint32_t t,
+                                                  uint32_t stride, uint32_t n1, uint32_t base_index,
+                                                  const uint2* w, uint32_t p) {
+#pragma unroll
+    for (int b = 0; b < levels; ++b) {
+        const uint32_t d = stride << b;
+        if (d < n1) continue;
+        const uint32_t shift = 31 - __clz(2 * d);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if (e & (1 << b)) continue;
+            const uint32_t jj = layout_index(layout, t, e);
+            const uint2 tw = w[(base_index >> shift) + (jj >> shift)];
+            gs_butterfly(x[e], x[e + (1 << b)], tw, p);
+        }
+    }
+}
+
+__device__ __forceinline__ void row_exchange(uint32_t* buf, uint32_t (&x)[8], int from, int to, uint32_t t) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) buf[layout_index(from, t, e)] = x[e];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = buf[layout_index(to, t, e)];
+    __syncthreads();
+}
+
+template <uint32_t N1>
+__global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ planes,
+                                                     const PrimeConst* __restrict__ consts,
+                                                     const uint2* __restrict__ twf, const uint2* __restrict__ twi,
+                                                     uint32_t ps, uint32_t n2, uint32_t rows, uint32_t r) {
+    constexpr uint32_t n1 = N1;
+    __shared__ uint32_t buf[2][kRowC];
+    const uint32_t row = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
+    const uint32_t p = consts[k].p, pinv = consts[k].pinv;
+    uint32_t* pa = planes + size_t(k) * ps + size_t(row) * kRowC;
+    const uint32_t* pb = planes + (size_t(r) + k) * ps + size_t(row) * kRowC;
+    const uint2* wf = twf + size_t(k) * n2;
+    const uint2* wi = twi + size_t(k) * n2;
+    // Table index of the group holding row-local jj at distance d:
+    // (rows + row) * (C / 2d) + jj / 2d  ==  ((rows + row) * C + jj) >> log2(2d).
+    const uint32_t base_index = (rows + row) * kRowC;
+    uint32_t xa[8], xb[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        xa[e] = pa[t + 256 * e];
+        xb[e] = pb[t + 256 * e];
+    }
+    row_forward_group(xa, 0, 3, t, 256, n1, base_index, wf, p);
+    row_forward_group(xb, 0, 3, t, 256, n1, base_index, wf, p);
+    row_exchange(buf[0], xa, 0, 1, t);
+    row_exchange(buf[1], xb, 0, 1, t);
+    row_forward_group(xa, 1, 3, t, 32, n1, base_index, wf, p);
+    row_forward_group(xb, 1, 3, t, 32, n1, base_index, wf, p);
+    row_exchange(buf[0], xa, 1, 2, t);
+    row_exchange(buf[1], xb, 1, 2, t);
+    row_forward_group(xa, 2, 3, t, 4, n1, base_index, wf, p);
+    row_forward_group(xb, 2, 3, t, 4, n1, base_index, wf, p);
+    row_exchange(buf[0], xa, 2, 3, t);
+    row_exchange(buf[1], xb, 2, 3, t);
+    row_forward_group(xa, 3, 2, t, 1, n1, base_index, wf, p);
+    row_forward_group(xb, 3, 2, t, 1, n1, base_index, wf, p);
+
+    // base case: thread t holds jj = 8t .. 8t+7, i.e. 8/n1 whole blocks
+    uint32_t xc[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        constexpr int n1i = int(N1);
+        const int s1 = s % n1i, blk0 = s - s1;  // block start inside the 8 values
+        uint32_t lo = 0, hi = 0;
+#pragma unroll
+        for (int u = 0; u < n1i; ++u) {
+            const uint32_t prod = mont(xa[blk0 + u], xb[blk0 + (s1 >= u ? s1 - u : s1 + n1i - u)], p, pinv);
+            if (u <= s1) lo = add_mod(lo, prod, p);
+            else hi = add_mod(hi, prod, p);
+        }
+        uint32_t value = lo;
+        if (n1 > 1) {
+            const uint32_t gb = (row * kRowC + 8 * t + blk0) / n1;  // global block index
+            if (n2 == 1) {
+                value = sub_mod(lo, hi, p);
+            } else {
+                const uint2 w = wf[n2 / 2 + gb / 2];
+                const uint32_t z = mul_shoup(hi, w.x, w.y, p);
+                value = (gb & 1) ? sub_mod(lo, z, p) : add_mod(lo, z, p);
+            }
+        }
+        xc[s] = value;
+    }
+
+    row_inverse_group(xc, 3, 2, t, 1, n1, base_index, wi, p);
+    row_exchange(buf[0], xc, 3, 2, t);
+    row_inverse_group(xc, 2, 3, t, 4, n1, base_index, wi, p);
+    row_exchange(buf[0], xc, 2, 1, t);
+    row_inverse_group(xc, 1, 3, t, 32, n1, base_index, wi, p);
+    row_exchange(buf[0], xc, 1, 0, t);
+    row_inverse_group(xc, 0, 3, t, 256, n1, base_index, wi, p);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) pa[t + 256 * e] = xc[e];
+}
+
+// ---------------------------------------------------------------------------
 // 6. reconstruction: one lane per coefficient, words in [j][n_pad] layout.
 // ---------------------------------------------------------------------------
 struct ReconstructArgs {
@@ -591,40 +795,87 @@ void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* pack
                        hipStream_t stream) {
     const uint32_t blocks = (s.conv_rows / kTile) * 2 * (s.n_pad / kTile);
     hipLaunchKernelGGL(conversion_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wconv, packed,
-                       t.consts, planes, s.n, s.n_pad, s.r, s.conv_rows, s.conv_k / kTK);
+                       t.consts, planes, s.n, s.n_pad, s.r, s.r_pad, s.conv_rows, s.conv_k / kTK);
     check_launch("conversion_gemm");
 }
 
 void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream) {
     if (s.rows < 2) return;
-    const uint32_t cw = column_width(s);
-    dim3 grid(s.row_width / cw, s.r, 2);
-    hipLaunchKernelGGL(ntt_columns_forward, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes, t.consts,
-                       t.tw_fwd, s.n_pad, s.n2, s.row_width, s.rows, cw, s.r);
+    const uint32_t ps = s.n_pad, r = s.r_pad, C = s.row_width;
+    if (C % 256 == 0 && s.rows <= 64) {
+        dim3 grid(C / 256, s.r, 2);
+        switch (s.rows) {
+#define PM_FWD(R)                                                                                                \
+    case R:                                                                                                      \
+        hipLaunchKernelGGL(ntt_columns_forward_reg<R>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_fwd, ps, \
+                           s.n2, C, r);                                                                          \
+        break;
+            PM_FWD(2) PM_FWD(4) PM_FWD(8) PM_FWD(16) PM_FWD(32) PM_FWD(64)
+#undef PM_FWD
+        }
+    } else {
+        const uint32_t cw = column_width(s);
+        dim3 grid(C / cw, s.r, 2);
+        hipLaunchKernelGGL(ntt_columns_forward, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes, t.consts,
+                           t.tw_fwd, ps, s.n2, C, s.rows, cw, r);
+    }
     check_launch("ntt_columns_forward");
 }
 
 void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream) {
+    const uint32_t ps = s.n_pad, r = s.r_pad;
     dim3 grid(s.rows, s.r);
-    hipLaunchKernelGGL(ntt_rows_product, grid, dim3(256), size_t(2) * s.row_width * 4, stream, planes, t.consts,
-                       t.tw_fwd, t.tw_inv, s.n_pad, s.n1, s.n2, s.row_width, s.rows, s.r);
+    if (s.row_width == kRowC && s.n1 <= 8) {
+        switch (s.n1) {
+#define PM_ROWS(B)                                                                                              \
+    case B:                                                                                                     \
+        hipLaunchKernelGGL(ntt_rows_2048<B>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_fwd, t.tw_inv, ps, \
+                           s.n2, s.rows, r);                                                                    \
+        break;
+            PM_ROWS(1) PM_ROWS(2) PM_ROWS(4) PM_ROWS(8)
+#undef PM_ROWS
+        }
+    } else {
+        hipLaunchKernelGGL(ntt_rows_product, grid, dim3(256), size_t(2) * s.row_width * 4, stream, planes, t.consts,
+                           t.tw_fwd, t.tw_inv, ps, s.n1, s.n2, s.row_width, s.rows, r);
+    }
     check_launch("ntt_rows_product");
 }
 
-void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream) {
-    if (s.rows < 2) return;
-    const uint32_t cw = column_width(s);
-    dim3 grid(s.row_width / cw, s.r);
-    hipLaunchKernelGGL(ntt_columns_inverse, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes, t.consts,
-                       t.tw_inv, s.n_pad, s.n2, s.row_width, s.rows, cw);
-    check_launch("ntt_columns_inverse");
+void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
+                                hipStream_t stream) {
+    const uint32_t ps = s.n_pad, ksteps = s.r_pad / 16;
+    if (s.row_width % 16 == 0 && s.rows <= 64) {
+        dim3 grid(s.row_width / 16, s.r_pad / 16);
+        switch (s.rows) {
+#define PM_INV(R)                                                                                              \
+    case R:                                                                                                    \
+        hipLaunchKernelGGL(ntt_columns_inverse_tiles<R>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_inv, \
+                           ps, s.n, s.n2, s.row_width, s.r, ksteps, rtiles);                                   \
+        break;
+            PM_INV(1) PM_INV(2) PM_INV(4) PM_INV(8) PM_INV(16) PM_INV(32) PM_INV(64)
+#undef PM_INV
+        }
+    } else {
+        // large or narrow shapes: LDS column stages, then a pure transpose into the tiles
+        if (s.rows >= 2) {
+            const uint32_t cw = column_width(s);
+            dim3 grid(s.row_width / cw, s.r);
+            hipLaunchKernelGGL(ntt_columns_inverse, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes,
+                               t.consts, t.tw_inv, ps, s.n2, s.row_width, s.rows, cw);
+            check_launch("ntt_columns_inverse");
+        }
+        dim3 grid((s.n + 15) / 16, s.r_pad / 16);
+        hipLaunchKernelGGL(ntt_columns_inverse_tiles<1>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_inv, ps,
+                           s.n, s.n2, s.n, s.r, ksteps, rtiles);
+    }
+    check_launch("ntt_columns_inverse_tiles");
 }
 
-void launch_crt(const Shape& s, const DeviceTables& t, const uint32_t* product_plane, int64_t* words,
-                hipStream_t stream) {
+void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream) {
     const uint32_t blocks = (s.crt_rows / kTile) * (s.n_pad / kTile);
-    hipLaunchKernelGGL(crt_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, product_plane, words,
-                       s.n, s.n_pad, s.r, s.crt_rows, s.crt_k / kTK, s.crt_words);
+    hipLaunchKernelGGL(crt_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words, s.n,
+                       s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words);
     check_launch("crt_gemm");
 }
 

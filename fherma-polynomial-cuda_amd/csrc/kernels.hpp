@@ -25,9 +25,10 @@ struct DeviceTables {
 // Working buffers (row stride n_pad).
 struct DeviceWork {
     int8_t* packed = nullptr;    // [2][n_pad/256][conv_k/64][256][64] signed operand bytes
-    uint32_t* planes = nullptr;  // [2][r][n_pad] residues; plane 0 receives v_k
+    uint32_t* planes = nullptr;  // [2][r_pad][n_pad] residues of a and b
+    int8_t* rtiles = nullptr;    // CRT operand tiles of v_k (aliases the b plane, dead after the row stages)
     int64_t* words = nullptr;    // [crt_words][n_pad] CRT word sums
-    uint32_t* t_words = nullptr; // [t_words][n_pad] T scratch (aliases the second plane)
+    uint32_t* t_words = nullptr; // [t_words][n_pad] T scratch (aliases the a plane, dead after the CRT GEMM)
 };
 
 // One-time kernel attributes (dynamic LDS limits); call before any launch.
@@ -40,9 +41,11 @@ void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* pack
                        hipStream_t stream);
 void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
 void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
-void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
-void launch_crt(const Shape& s, const DeviceTables& t, const uint32_t* product_plane, int64_t* words,
-                hipStream_t stream);
+// Inverse row-spanning stages of plane 0; writes v_k into the CRT operand tiles
+// [n_pad/256][r_pad/16][256][64] (signed bytes).
+void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
+                                hipStream_t stream);
+void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream);
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
                         uint32_t* c, hipStream_t stream);
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream);
