@@ -105,6 +105,8 @@ def main():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--no-secondary", action="store_true", help="skip the reference-point and e2e legs")
     parser.add_argument("--cpu-sample", type=int, default=8192)
+    parser.add_argument("--calibrate-copy", action="store_true",
+                        help="also run one plain 1 GiB device-to-device copy (counter calibration)")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: one product runs on one GPU; the multi-GPU "
@@ -113,6 +115,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give
+        # the scale that turns those counters into HBM bytes
+        src = torch.empty(1 << 28, dtype=torch.int32, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     polymul = _load("polymul")
     workloads = _load("workloads")

@@ -1,0 +1,52 @@
+"""The C ABI library loads on a machine without a GPU and exports every entry
+point the headers under include/ declare; the fherma shim exports the three
+harness functions; the case runner starts and reports usage.  No compute
+calls are made here."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "fherma-polynomial-cuda_amd"
+
+
+def _declared_c_functions():
+    text = (ROOT / "include" / "polymul.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:pm_status|void|const char\s*\*)\s*(pm_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_function(polymul):
+    lib = ctypes.CDLL(str(polymul.LIBRARY_PATH))
+    names = _declared_c_functions()
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_abi(polymul):
+    lib = polymul.load_library()
+    for name in _declared_c_functions():
+        assert getattr(lib, name).restype is not None or name == "pm_plan_destroy"
+
+
+def test_fherma_shim_exports_the_harness_entry_points():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(PKG / "libfherma_amd.so")], capture_output=True,
+                         text=True, check=True).stdout
+    demangled = subprocess.run(["c++filt"], input=out, capture_output=True, text=True, check=True).stdout
+    for signature in ("fherma_init(fherma::Point const&)", "fherma_run(void*, fherma::Inputs const&)",
+                      "fherma_free(void*)"):
+        assert signature in demangled, signature
+
+
+def test_case_runner_usage():
+    proc = subprocess.run([str(PKG / "solution")], capture_output=True, text=True)
+    assert proc.returncode == 2 and "usage" in proc.stderr
+
+
+def test_last_error_is_thread_local_and_readable(polymul):
+    lib = polymul.load_library()
+    handle = ctypes.c_void_p()
+    status = lib.pm_plan_create_host(3, 7, 1, (ctypes.c_uint32 * 1)(97), ctypes.byref(handle))
+    assert status == 1
+    assert b"power of two" in lib.pm_last_error()

@@ -1,0 +1,241 @@
+"""Host-side plan (the work of fherma_init) checked on the CPU through the C ABI
+(pm_plan_create_host: tables only, no GPU).  Each table is compared with its
+closed form computed here with Python integers, and the whole arithmetic of
+the GPU path is replayed with numpy from the exported tables alone and
+compared limb-for-limb with the oracle."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import sympy
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+R32 = 1 << 32
+
+
+def _plan(polymul, oracle, n, width, q):
+    limbs = (width + 31) // 32
+    return polymul.Plan(n, width, limbs, oracle.int_to_words(q, limbs), device=False)
+
+
+def _random_q(workloads, limbs, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return workloads.words_to_int(workloads.random_modulus(limbs, rng))
+
+
+def _bitrev(x, bits):
+    return int(format(x, "0%db" % bits)[::-1], 2) if bits else 0
+
+
+def test_reference_points_use_the_reference_walk(polymul, oracle):
+    recorded = json.loads((GOLDEN / "ref_primes.json").read_text())["reports"]
+    points = json.loads((GOLDEN / "points.json").read_text())
+    for rep in recorded:
+        width = 1747 if rep["N"] == 65536 else 868
+        q = int(points["%d/%d" % (rep["N"], width)]["q"], 16)
+        with _plan(polymul, oracle, rep["N"], width, q) as plan:
+            info = plan.info()
+            assert info["reference_walk"] == 1 and info["block"] == 1
+            assert [int(p) for p in plan.primes()] == rep["primes"]
+
+
+@pytest.mark.parametrize("name", ["mid", "large"])
+def test_wide_moduli_split_the_transform(polymul, oracle, workloads, name):
+    cfg = workloads.CONFIGS[name]
+    n, limbs = cfg["n"], cfg["limbs"]
+    q = _random_q(workloads, limbs, cfg["seed"])
+    with pytest.raises(ValueError):
+        oracle.reference_primes(n, q)  # solve.cu:910 "insufficient independent NTT fields"
+    with _plan(polymul, oracle, n, 32 * limbs, q) as plan:
+        info = plan.info()
+        n1, n2 = info["block"], info["transform"]
+        assert n1 > 1 and n1 * n2 == n and info["reference_walk"] == 0
+        primes = [int(p) for p in plan.primes()]
+        assert primes == oracle.reference_primes(n, q, step=2 * n2)
+        with pytest.raises(ValueError):  # the next smaller split does not suffice
+            oracle.reference_primes(n, q, step=4 * n2)
+        assert len(set(primes)) == len(primes)
+        assert all((p - 1) % (2 * n2) == 0 and (1 << 30) <= p < (1 << 31) for p in primes)
+        assert all(sympy.isprime(p) for p in primes[::97])
+        product = 1
+        for p in primes:
+            product *= p
+        assert product > 4 * n * (q - 1) ** 2
+
+
+CASES = [(64, 33, "ntt"), (1024, 256, "random"), (256, 1760, "random"), (8, 64, "random"), (2048, 100, "ntt")]
+
+
+def _case_q(oracle, workloads, n, width, kind):
+    if kind == "ntt":
+        return oracle.modulus(n, width)
+    return _random_q(workloads, (width + 31) // 32, n * 7919 + width)
+
+
+@pytest.mark.parametrize("n,width,kind", CASES)
+def test_prime_constants_and_conversion_weights(polymul, oracle, workloads, n, width, kind):
+    q = _case_q(oracle, workloads, n, width, kind)
+    with _plan(polymul, oracle, n, width, q) as plan:
+        s = plan.shape()
+        primes = [int(p) for p in plan.primes()]
+        r, limbs, n2 = s["r"], s["limbs"], s["n2"]
+        consts = plan.table(polymul.TABLE_PRIME_CONSTS, np.uint32).reshape(r, 4)
+        beta = plan.table(polymul.TABLE_BETA, np.uint32)
+        w = plan.table(polymul.TABLE_CONV_WEIGHTS, np.int8).reshape(2, s["conv_rows"], s["conv_k"])
+        product = 1
+        for p in primes:
+            product *= p
+        for k, p in enumerate(primes):
+            assert int(consts[k, 0]) == p
+            assert int(consts[k, 1]) * p % R32 == 1
+            pk_inv = pow((product // p) % p, -1, p)
+            assert int(beta[k]) == pow(n2, -1, p) * R32 * pk_inv % p
+            for op, lam in ((0, R32 % p), (1, R32 * int(beta[k]) % p)):
+                rows = w[op, 4 * k:4 * k + 4, :].astype(np.int64)
+                values = rows[0] + 256 * rows[1] + 65536 * rows[2] + 16777216 * rows[3]
+                for j in range(4 * limbs):
+                    assert int(values[j]) % p == lam * pow(256, j, p) % p
+                assert not rows[:, 4 * limbs:].any()
+                off = int(consts[k, 2 + op])
+                assert off == 128 * int(values[:4 * limbs].sum()) % p
+        assert not w[:, 4 * r:, :].any()
+
+
+@pytest.mark.parametrize("n,width,kind", CASES)
+def test_twiddles(polymul, oracle, workloads, n, width, kind):
+    q = _case_q(oracle, workloads, n, width, kind)
+    with _plan(polymul, oracle, n, width, q) as plan:
+        s = plan.shape()
+        r, n2, log_n2 = s["r"], s["n2"], s["log_n2"]
+        fwd = plan.table(polymul.TABLE_TWIDDLES_FWD, np.uint32).reshape(r, n2, 2)
+        inv = plan.table(polymul.TABLE_TWIDDLES_INV, np.uint32).reshape(r, n2, 2)
+        for k, p in enumerate(int(x) for x in plan.primes()):
+            psi = int(fwd[k, n2 // 2, 0]) if n2 > 1 else p - 1
+            assert pow(psi, n2, p) == p - 1  # primitive 2*n2-th root
+            for j in range(0, n2, max(1, n2 // 64)):
+                wf, wi = int(fwd[k, j, 0]), int(inv[k, j, 0])
+                assert wf == pow(psi, _bitrev(j, log_n2), p)
+                assert wf * wi % p == 1
+                assert int(fwd[k, j, 1]) == (wf << 32) // p and int(inv[k, j, 1]) == (wi << 32) // p
+
+
+@pytest.mark.parametrize("n,width,kind", CASES)
+def test_crt_tables(polymul, oracle, workloads, n, width, kind):
+    q = _case_q(oracle, workloads, n, width, kind)
+    with _plan(polymul, oracle, n, width, q) as plan:
+        s = plan.shape()
+        primes = [int(p) for p in plan.primes()]
+        r, limbs = s["r"], s["limbs"]
+        product = 1
+        for p in primes:
+            product *= p
+        m = plan.table(polymul.TABLE_M_MOD_Q, np.uint32).reshape(r, limbs)
+        m_vals = [oracle.words_to_int(m[k]) for k in range(r)]
+        for k, p in enumerate(primes):
+            assert m_vals[k] == (product // p) % q
+        assert oracle.words_to_int(plan.table(polymul.TABLE_NEG_P_MOD_Q, np.uint32)) == (-product) % q
+        rcp = [int(x) for x in plan.table(polymul.TABLE_RECIPROCALS, np.uint64)]
+        assert rcp == [((1 << 64) - 1) // p for p in primes]
+        assert oracle.words_to_int(plan.table(polymul.TABLE_CORR_T, np.uint32)) == 128 * 0x01010101 * sum(m_vals)
+        assert oracle.words_to_int(plan.table(polymul.TABLE_CORR_E, np.uint32)) == 128 * 0x01010101 * sum(rcp)
+        wc = plan.table(polymul.TABLE_CRT_WEIGHTS, np.int8).reshape(s["crt_rows"], s["crt_k"]).astype(np.int64)
+        t_rows = 4 * (limbs + 1)
+        scale = [1 << (8 * c) for c in range(t_rows)]
+        escale = [1 << (8 * c) for c in range(12)]
+        for k in range(0, r, max(1, r // 8)):
+            for d in range(4):
+                col = wc[:, 4 * k + d]
+                # byte convolution: sum_c 256^c w[c][4k+d] == 256^d * M_k
+                assert sum(int(col[c]) * scale[c] for c in range(t_rows)) == m_vals[k] << (8 * d)
+                assert sum(int(col[t_rows + c]) * escale[c] for c in range(12)) == rcp[k] << (8 * d)
+        h = q.bit_length() - 1
+        assert s["mu"] == (1 << (h + 63)) // q
+        assert s["s_shift"] == max(h - 16, 0) and s["k_shift"] == h - s["s_shift"] + 63
+
+
+def _redc_signed(value, p):
+    return value * pow(R32, -1, p) % p
+
+
+@pytest.mark.parametrize("n,width,kind", [(64, 33, "ntt"), (256, 1760, "random"), (8, 64, "random")])
+def test_pipeline_restatement_from_tables(polymul, oracle, workloads, n, width, kind):
+    """Replays the GPU arithmetic with numpy from the exported tables: signed-byte
+    conversion GEMM + REDC, v_k from the exact product, signed-byte CRT GEMM,
+    quotient from the reciprocal columns, carries and the Barrett reduction;
+    the result must equal the oracle's product."""
+    q = _case_q(oracle, workloads, n, width, kind)
+    limbs = (width + 31) // 32
+    rng = np.random.Generator(np.random.PCG64(n + width))
+    qw = oracle.int_to_words(q, limbs)
+    a = workloads.uniform_below(qw, n, rng)
+    b = workloads.uniform_below(qw, n, rng)
+    with _plan(polymul, oracle, n, width, q) as plan:
+        s = plan.shape()
+        primes = [int(p) for p in plan.primes()]
+        r = s["r"]
+        consts = plan.table(polymul.TABLE_PRIME_CONSTS, np.uint32).reshape(r, 4)
+        beta = [int(x) for x in plan.table(polymul.TABLE_BETA, np.uint32)]
+        w = plan.table(polymul.TABLE_CONV_WEIGHTS, np.int8).reshape(2, s["conv_rows"], s["conv_k"]).astype(np.int64)
+        # conversion: signed operand bytes (u-128) x signed weight digits
+        for op, x in ((0, a), (1, b)):
+            bytes_ = np.zeros((n, s["conv_k"]), dtype=np.int64)
+            bytes_[:, :4 * limbs] = x.view(np.uint8).reshape(n, 4 * limbs)
+            signed = bytes_ - 128
+            g = w[op] @ signed.T  # [conv_rows][n], exact in int64
+            for k, p in enumerate(primes[:8]):
+                sums = g[4 * k] + 256 * g[4 * k + 1] + 65536 * g[4 * k + 2] + 16777216 * g[4 * k + 3]
+                got = [_redc_signed(int(v) + int(consts[k, 2 + op]), p) for v in sums]
+                vals = oracle.from_limbs(x)
+                lam = 1 if op == 0 else beta[k]
+                assert got == [lam * v % p for v in vals]
+        # v_k from the exact signed product, then the CRT GEMM on signed bytes
+        xs = oracle.signed_convolution(oracle.from_limbs(a), oracle.from_limbs(b))
+        product = 1
+        for p in primes:
+            product *= p
+        v = np.zeros((n, s["crt_k"]), dtype=np.int64)
+        vb = np.zeros((n, s["r_pad"]), dtype=np.uint32)
+        for k, p in enumerate(primes):
+            inv = pow((product // p) % p, -1, p)
+            vb[:, k] = [(xi % p) * inv % p for xi in xs]
+        v[:, :] = vb.view(np.uint8).reshape(n, 4 * s["r_pad"]).astype(np.int64) - 128
+        wc = plan.table(polymul.TABLE_CRT_WEIGHTS, np.int8).reshape(s["crt_rows"], s["crt_k"]).astype(np.int64)
+        g = wc @ v.T  # [crt_rows][n]
+        words = g[0::4] + 256 * g[1::4] + 65536 * g[2::4] + 16777216 * g[3::4]  # [crt_words_pad][n]
+        corr_t = oracle.words_to_int(plan.table(polymul.TABLE_CORR_T, np.uint32))
+        corr_e = oracle.words_to_int(plan.table(polymul.TABLE_CORR_E, np.uint32))
+        neg_p = oracle.words_to_int(plan.table(polymul.TABLE_NEG_P_MOD_Q, np.uint32))
+        out = []
+        for i in range(n):
+            e = sum(int(words[limbs + 1 + j, i]) << (32 * j) for j in range(3)) + corr_e
+            kq = (e + (1 << 63)) >> 64
+            t = sum(int(words[j, i]) << (32 * j) for j in range(limbs + 1)) + corr_t + kq * neg_p
+            assert 0 <= t < (1 << (32 * (limbs + 2)))
+            ts = t >> s["s_shift"]
+            qe = (ts * s["mu"]) >> s["k_shift"]
+            rem = t - qe * q
+            assert 0 <= rem < 2 * q  # the Barrett estimate is at most one short
+            out.append(rem - q if rem >= q else rem)
+        assert out == oracle.kronecker(oracle.from_limbs(a), oracle.from_limbs(b), q)
+
+
+def test_invalid_points_and_host_only_plans(polymul, oracle):
+    q = oracle.int_to_words(97, 1)
+    for n, width, limbs in ((3, 7, 1), (0, 7, 1), (8, 7, 2), (8, 40, 1)):
+        with pytest.raises(polymul.PolymulError) as err:
+            polymul.Plan(n, width, limbs, q if limbs == 1 else np.zeros(limbs, np.uint32), device=False)
+        assert err.value.status == 1  # PM_ERR_INVALID_ARGUMENT
+    with pytest.raises(polymul.PolymulError) as err:
+        polymul.Plan(8, 7, 1, np.zeros(1, np.uint32), device=False)
+    assert err.value.status == 1
+    with pytest.raises(polymul.PolymulError) as err:
+        polymul.Plan(1 << 28, 64, 2, oracle.int_to_words((1 << 61) - 1, 2), device=False)
+    assert err.value.status == 1  # transform beyond the supported size
+    with polymul.Plan(8, 1, 1, oracle.int_to_words(1, 1), device=False) as plan:
+        assert plan.info()["trivial"] == 1  # q == 1: the zero ring
+    with polymul.Plan(8, 16, 1, oracle.int_to_words(65521, 1), device=False) as plan:
+        a = np.zeros((8, 1), np.uint32)
+        with pytest.raises(polymul.PolymulError) as err:
+            plan.multiply(a, a)
+        assert err.value.status == 6  # PM_ERR_NO_DEVICE
