@@ -9,12 +9,15 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <algorithm>
 #include <memory>
+#include <thread>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "copy_pool.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
@@ -36,6 +39,9 @@ void hip_check(hipError_t e, const char* where) {
 // Events bracket the 7 stage launches; times are the 7 intervals plus the total.
 constexpr int kEvents = 8;
 constexpr int kStages = 8;
+// The host-vector path moves operands in and results out in up to this many
+// coefficient chunks, overlapping copies, transfers and the per-coefficient stages.
+constexpr int kMaxChunks = 8;
 
 }  // namespace
 
@@ -50,6 +56,12 @@ struct pm_plan {
     uint32_t* io_a = nullptr;  // device copies for the host-vector entry point
     uint32_t* io_b = nullptr;
     uint32_t* io_c = nullptr;
+    // host staging for pm_multiply (created on first use)
+    uint32_t* pin_in = nullptr;   // [2][N][L] pinned
+    uint32_t* pin_out = nullptr;  // [N][L] pinned
+    hipStream_t copy_in = nullptr, copy_out = nullptr;
+    hipEvent_t in_ready[kMaxChunks] = {}, out_ready[kMaxChunks] = {}, reconstructed[kMaxChunks] = {};
+    std::unique_ptr<pm::CopyPool> pool;
     uint64_t device_bytes = 0;
     bool profiling = false;
     hipEvent_t events[kEvents] = {};
@@ -80,12 +92,107 @@ struct pm_plan {
     ~pm_plan() {
         if (device) {
             hipSetDevice(device_id);
-            if (stream) hipStreamSynchronize(stream);
+            for (hipStream_t st : {stream, copy_in, copy_out})
+                if (st) hipStreamSynchronize(st);
             for (hipEvent_t& e : events)
                 if (e) hipEventDestroy(e);
+            for (int i = 0; i < kMaxChunks; ++i)
+                for (hipEvent_t e : {in_ready[i], out_ready[i], reconstructed[i]})
+                    if (e) hipEventDestroy(e);
             for (void* p : allocations) hipFree(p);
-            if (stream) hipStreamDestroy(stream);
+            if (pin_in) hipHostFree(pin_in);
+            if (pin_out) hipHostFree(pin_out);
+            for (hipStream_t st : {stream, copy_in, copy_out})
+                if (st) hipStreamDestroy(st);
         }
+    }
+
+    void make_staging() {
+        if (pin_in) return;
+        const size_t bytes = size_t(host.shape.n) * host.shape.limbs * sizeof(uint32_t);
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pin_in), 2 * bytes, hipHostMallocDefault),
+                  "pin input staging");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pin_out), bytes, hipHostMallocDefault),
+                  "pin output staging");
+        hip_check(hipStreamCreateWithFlags(&copy_in, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&copy_out, hipStreamNonBlocking), "hipStreamCreate");
+        for (int i = 0; i < kMaxChunks; ++i) {
+            hip_check(hipEventCreateWithFlags(&in_ready[i], hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&out_ready[i], hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&reconstructed[i], hipEventDisableTiming), "hipEventCreate");
+        }
+        unsigned hw = std::thread::hardware_concurrency();
+        pool = std::make_unique<pm::CopyPool>(hw ? std::min(16u, hw) : 4u);
+    }
+
+    // Host vectors in, host vector out.  Chunk c (256-coefficient tiles
+    // [t0, t0+tc)) is copied into pinned memory by the pool, sent on copy_in,
+    // then packed and converted on the compute stream while the next chunk is
+    // still being copied; after the transforms and the CRT GEMM, results are
+    // reconstructed chunk by chunk and each chunk's D2H and host copy-out
+    // overlap the next chunk's reconstruction.
+    // b == nullptr: the b residues come from `bplanes` (a resident operand).
+    void multiply_host(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in, uint32_t* c) {
+        const pm::Shape& s = host.shape;
+        const uint32_t nops = b ? 2 : 1;
+        const uint32_t* bplanes = b ? b_planes() : bplanes_in;
+        const size_t L = s.limbs, row_bytes = L * sizeof(uint32_t);
+        if (s.trivial) {
+            std::memset(c, 0, size_t(s.n) * row_bytes);
+            return;
+        }
+        make_staging();
+        const uint32_t ntiles = s.n_pad / 256;
+        const uint32_t chunks = ntiles < kMaxChunks ? ntiles : kMaxChunks;
+        const uint32_t per = (ntiles + chunks - 1) / chunks;
+        uint32_t* pin_a = pin_in;
+        uint32_t* pin_b = pin_in + size_t(s.n) * L;
+        auto rows_of = [&](uint32_t ch, size_t& r0, size_t& r1) {
+            r0 = std::min<size_t>(size_t(ch) * per * 256, s.n);
+            r1 = std::min<size_t>(size_t(ch + 1) * per * 256, s.n);
+        };
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            size_t r0, r1;
+            rows_of(ch, r0, r1);
+            if (r1 > r0) {
+                const size_t off = r0 * L, bytes = (r1 - r0) * row_bytes;
+                pool->copy(pin_a + off, a + off, bytes);
+                hip_check(hipMemcpyAsync(io_a + off, pin_a + off, bytes, hipMemcpyHostToDevice, copy_in), "H2D a");
+                if (b) {
+                    pool->copy(pin_b + off, b + off, bytes);
+                    hip_check(hipMemcpyAsync(io_b + off, pin_b + off, bytes, hipMemcpyHostToDevice, copy_in),
+                              "H2D b");
+                }
+            }
+            hip_check(hipEventRecord(in_ready[ch], copy_in), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(stream, in_ready[ch], 0), "hipStreamWaitEvent");
+            const uint32_t t0 = ch * per, tc = std::min(per, ntiles - t0);
+            if (tc) {
+                pm::launch_pack(s, io_a, io_b, work.packed, t0, tc, 0, nops, stream);
+                pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
+            }
+        }
+        run_middle(stream, nops, bplanes);
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            size_t r0, r1;
+            rows_of(ch, r0, r1);
+            pm::launch_reconstruct(s, tables, work.words, work.t_words, io_c, uint32_t(r0), uint32_t(r1 - r0),
+                                   stream);
+            hip_check(hipEventRecord(reconstructed[ch], stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(copy_out, reconstructed[ch], 0), "hipStreamWaitEvent");
+            if (r1 > r0)
+                hip_check(hipMemcpyAsync(pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
+                                         hipMemcpyDeviceToHost, copy_out),
+                          "D2H c");
+            hip_check(hipEventRecord(out_ready[ch], copy_out), "hipEventRecord");
+        }
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            size_t r0, r1;
+            rows_of(ch, r0, r1);
+            hip_check(hipEventSynchronize(out_ready[ch]), "D2H c");
+            if (r1 > r0) pool->copy(c + r0 * L, pin_out + r0 * L, (r1 - r0) * row_bytes);
+        }
+        hip_check(hipStreamSynchronize(stream), "multiply");
     }
 
     void make_device() {
@@ -129,10 +236,18 @@ struct pm_plan {
         work.rtiles = reinterpret_cast<int8_t*>(work.planes + half);
         work.t_words = work.planes;
         work.words = alloc<int64_t>(size_t(s.crt_words) * s.n_pad);
+        // pinned staging, streams and copy workers of the host entry points are
+        // set up here, with the rest of the plan, not inside the first product
+        make_staging();
     }
 
-    // Runs stages up to and including `last` (PM_STAGE_*) on device operands.
-    void run(const uint32_t* a, const uint32_t* b, uint32_t* c, int last, hipStream_t st) {
+    // The b residues inside the work planes ([r_pad][n_pad] after a's).
+    uint32_t* b_planes() { return work.planes + size_t(host.shape.r_pad) * host.shape.n_pad; }
+
+    // Runs stages up to and including `last` (PM_STAGE_*) on device operands;
+    // b == nullptr takes the b residues from `bplanes_in` (a resident operand).
+    void run(const uint32_t* a, const uint32_t* b, uint32_t* c, int last, hipStream_t st,
+             const uint32_t* bplanes_in = nullptr) {
         const pm::Shape& s = host.shape;
         if (s.trivial) {
             pm::launch_zero(c, size_t(s.n) * s.limbs, st);
@@ -141,15 +256,18 @@ struct pm_plan {
         auto mark = [&](int i) {
             if (profiling) hip_check(hipEventRecord(events[i], st), "hipEventRecord");
         };
+        const uint32_t ntiles = s.n_pad / 256;
+        const uint32_t nops = b ? 2 : 1;
+        const uint32_t* bplanes = b ? b_planes() : bplanes_in;
         mark(0);
-        pm::launch_pack(s, a, b, work.packed, st);
+        pm::launch_pack(s, a, b, work.packed, 0, ntiles, 0, nops, st);
         mark(1);
-        pm::launch_conversion(s, tables, work.packed, work.planes, st);
+        pm::launch_conversion(s, tables, work.packed, work.planes, 0, ntiles, 0, nops, st);
         mark(2);
         if (last == PM_STAGE_RESIDUES) return;
-        pm::launch_ntt_forward_columns(s, tables, work.planes, st);
+        pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
         mark(3);
-        pm::launch_ntt_rows_product(s, tables, work.planes, st);
+        pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
         mark(4);
         pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
         mark(5);
@@ -157,8 +275,17 @@ struct pm_plan {
         pm::launch_crt(s, tables, work.rtiles, work.words, st);
         mark(6);
         if (last == PM_STAGE_WORDS) return;
-        pm::launch_reconstruct(s, tables, work.words, work.t_words, c, st);
+        pm::launch_reconstruct(s, tables, work.words, work.t_words, c, 0, s.n, st);
         mark(7);
+    }
+
+    // The stages between the per-coefficient ends: transforms and the CRT GEMM.
+    void run_middle(hipStream_t st, uint32_t nops, const uint32_t* bplanes) {
+        const pm::Shape& s = host.shape;
+        pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
+        pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
+        pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
+        pm::launch_crt(s, tables, work.rtiles, work.words, st);
     }
 
     void collect_times() {
@@ -166,6 +293,20 @@ struct pm_plan {
         for (int i = 0; i < kEvents - 1; ++i)
             hip_check(hipEventElapsedTime(&stage_ms[i], events[i], events[i + 1]), "hipEventElapsedTime");
         hip_check(hipEventElapsedTime(&stage_ms[kEvents - 1], events[0], events[kEvents - 1]), "hipEventElapsedTime");
+    }
+};
+
+// A resident factor: b's residues after conversion and the row-spanning
+// forward stages, [r][n_pad] u32 on the plan's device.  Valid only with the
+// plan that made it (b's weights carry that plan's scale beta_k).
+struct pm_operand {
+    pm_plan* plan = nullptr;
+    uint32_t* planes = nullptr;
+    ~pm_operand() {
+        if (planes) {
+            hipSetDevice(plan->device_id);
+            hipFree(planes);
+        }
     }
 };
 
@@ -246,15 +387,7 @@ pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint3
         require_device(plan);
         if (!a || !b || !c) throw std::invalid_argument("null operand");
         hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
-        const pm::Shape& s = plan->host.shape;
-        const size_t bytes = size_t(s.n) * s.limbs * sizeof(uint32_t);
-        hipStream_t st = plan->stream;
-        hip_check(hipMemcpyAsync(plan->io_a, a, bytes, hipMemcpyHostToDevice, st), "copy a to device");
-        hip_check(hipMemcpyAsync(plan->io_b, b, bytes, hipMemcpyHostToDevice, st), "copy b to device");
-        plan->run(plan->io_a, plan->io_b, plan->io_c, PM_STAGE_OUTPUT, st);
-        hip_check(hipMemcpyAsync(c, plan->io_c, bytes, hipMemcpyDeviceToHost, st), "copy c to host");
-        hip_check(hipStreamSynchronize(st), "multiply");
-        plan->collect_times();
+        plan->multiply_host(a, b, nullptr, c);
     });
 }
 
@@ -269,6 +402,67 @@ pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b
         plan->collect_times();
     });
 }
+
+pm_status pm_operand_create(pm_plan* plan, const uint32_t* b, pm_operand** out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null output pointer");
+        *out = nullptr;
+        require_device(plan);
+        if (!b) throw std::invalid_argument("null operand");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        const pm::Shape& s = plan->host.shape;
+        auto op = std::make_unique<pm_operand>();
+        op->plan = plan;
+        if (s.trivial) {
+            *out = op.release();
+            return;
+        }
+        const size_t plane_words = size_t(s.r) * s.n_pad;
+        void* mem = nullptr;
+        hipError_t e = hipMalloc(&mem, plane_words * sizeof(uint32_t));
+        if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+        hip_check(e, "hipMalloc");
+        op->planes = static_cast<uint32_t*>(mem);
+        hipStream_t st = plan->stream;
+        const size_t bytes = size_t(s.n) * s.limbs * sizeof(uint32_t);
+        const uint32_t ntiles = s.n_pad / 256;
+        hip_check(hipMemcpyAsync(plan->io_b, b, bytes, hipMemcpyHostToDevice, st), "copy b to device");
+        pm::launch_pack(s, nullptr, plan->io_b, plan->work.packed, 0, ntiles, 1, 1, st);
+        pm::launch_conversion(s, plan->tables, plan->work.packed, plan->work.planes, 0, ntiles, 1, 1, st);
+        pm::launch_ntt_forward_columns(s, plan->tables, plan->work.planes, 1, 1, st);
+        hip_check(hipMemcpyAsync(op->planes, plan->b_planes(), plane_words * sizeof(uint32_t),
+                                 hipMemcpyDeviceToDevice, st),
+                  "keep b residues");
+        hip_check(hipStreamSynchronize(st), "operand");
+        *out = op.release();
+    });
+}
+
+pm_status pm_multiply_operand(pm_plan* plan, const uint32_t* a, const pm_operand* b, uint32_t* c) {
+    return guarded([&] {
+        require_device(plan);
+        if (!a || !b || !c) throw std::invalid_argument("null operand");
+        if (b->plan != plan) throw std::invalid_argument("operand belongs to another plan");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        plan->multiply_host(a, nullptr, b->planes, c);
+    });
+}
+
+pm_status pm_multiply_operand_device(pm_plan* plan, const uint32_t* a, const pm_operand* b, uint32_t* c,
+                                     void* stream) {
+    return guarded([&] {
+        require_device(plan);
+        if (!a || !b || !c) throw std::invalid_argument("null operand");
+        if (b->plan != plan) throw std::invalid_argument("operand belongs to another plan");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->stream;
+        plan->run(a, nullptr, c, PM_STAGE_OUTPUT, st, b->planes);
+        hip_check(hipStreamSynchronize(st), "multiply");
+        plan->collect_times();
+    });
+}
+
+void pm_operand_destroy(pm_operand* operand) { delete operand; }
 
 pm_status pm_plan_get_info(const pm_plan* plan, pm_plan_info* info) {
     return guarded([&] {

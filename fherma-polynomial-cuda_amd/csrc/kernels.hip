@@ -196,25 +196,28 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
 // 0. operand packing: [N][L] u32 -> [2][n_pad/256][conv_k/64][256][64] bytes,
 //    each byte XOR 0x80 (u -> u-128 as int8); padding bytes are zero values.
 // ---------------------------------------------------------------------------
+// Processes coefficient tiles [tile0, tile0 + tiles) of both operands.
 __global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict__ a,
                                                      const uint32_t* __restrict__ b,
                                                      uint4* __restrict__ packed, uint32_t n, uint32_t limbs,
-                                                     uint32_t ksteps, uint32_t ntiles) {
+                                                     uint32_t ksteps, uint32_t ntiles, uint32_t tile0,
+                                                     uint32_t tiles, uint32_t op0, uint32_t nops) {
     const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const size_t per_op = size_t(ntiles) * ksteps * kTile * 4;
-    if (idx >= 2 * per_op) return;
-    const uint32_t op = uint32_t(idx / per_op);
+    const size_t per_op = size_t(tiles) * ksteps * kTile * 4;
+    if (idx >= nops * per_op) return;
+    const uint32_t op = op0 + uint32_t(idx / per_op);
     size_t rest = idx % per_op;
-    const uint32_t chunk = uint32_t(rest & 3), row = uint32_t((rest >> 2) & 255);
+    const uint32_t inner = uint32_t(rest & 1023), chunk = inner & 3, row = inner >> 2;
     rest >>= 10;
-    const uint32_t ks = uint32_t(rest % ksteps), tn = uint32_t(rest / ksteps);
+    const uint32_t ks = uint32_t(rest % ksteps), tn = tile0 + uint32_t(rest / ksteps);
     const uint32_t coef = tn * kTile + row, w0 = ks * 16 + chunk * 4;
     const uint32_t* src = (op ? b : a) + size_t(coef) * limbs;
     uint32_t v[4];
 #pragma unroll
     for (uint32_t c = 0; c < 4; ++c) v[c] = (coef < n && w0 + c < limbs) ? src[w0 + c] : 0u;
     const uint32_t flip = 0x80808080u;
-    packed[idx] = make_uint4(v[0] ^ flip, v[1] ^ flip, v[2] ^ flip, v[3] ^ flip);
+    packed[((size_t(op) * ntiles + tn) * ksteps + ks) * 1024 + inner] =
+        make_uint4(v[0] ^ flip, v[1] ^ flip, v[2] ^ flip, v[3] ^ flip);
 }
 
 // ---------------------------------------------------------------------------
@@ -226,14 +229,15 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
                                                        const PrimeConst* __restrict__ consts,
                                                        uint32_t* __restrict__ planes, uint32_t n, uint32_t n_pad,
                                                        uint32_t r, uint32_t r_pad, uint32_t conv_rows,
-                                                       uint32_t ksteps) {
+                                                       uint32_t ksteps, uint32_t tile0, uint32_t tiles,
+                                                       uint32_t op0, uint32_t nops) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave >> 2, wn = wave & 3;
     const uint32_t ntiles = n_pad / kTile, num_m = conv_rows / kTile;
     uint32_t tm, tn;
-    tile_coords(num_m, 2 * ntiles, tm, tn);
-    const uint32_t op = tn / ntiles, tcol = tn % ntiles;
+    tile_coords(num_m, nops * tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles) of nops operands
+    const uint32_t op = op0 + tn / tiles, tcol = tile0 + tn % tiles;
     const int8_t* a_panel = wtiles + (size_t(op) * num_m + tm) * ksteps * kTileBytes;
     TiledB bl{packed + (size_t(op) * ntiles + tcol) * ksteps * kTileBytes};
 
@@ -309,9 +313,9 @@ __global__ __launch_bounds__(256) void ntt_columns_forward(uint32_t* __restrict_
                                                            const PrimeConst* __restrict__ consts,
                                                            const uint2* __restrict__ tw, uint32_t ps,
                                                            uint32_t n2, uint32_t C, uint32_t rows,
-                                                           uint32_t cw, uint32_t r) {
+                                                           uint32_t cw, uint32_t r, uint32_t op0) {
     extern __shared__ uint32_t tile[];  // [rows][cw]
-    const uint32_t col0 = blockIdx.x * cw, k = blockIdx.y, op = blockIdx.z;
+    const uint32_t col0 = blockIdx.x * cw, k = blockIdx.y, op = op0 + blockIdx.z;
     const uint32_t p = consts[k].p;
     uint32_t* data = planes + (size_t(op) * r + k) * ps;
     const uint2* w = tw + size_t(k) * n2;
@@ -336,6 +340,7 @@ __global__ __launch_bounds__(256) void ntt_columns_forward(uint32_t* __restrict_
 }
 
 __global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ planes,
+                                                        const uint32_t* __restrict__ bplanes,
                                                         const PrimeConst* __restrict__ consts,
                                                         const uint2* __restrict__ twf,
                                                         const uint2* __restrict__ twi, uint32_t ps,
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ p
     const uint32_t row = blockIdx.x, k = blockIdx.y;
     const uint32_t p = consts[k].p, pinv = consts[k].pinv;
     uint32_t* pa = planes + size_t(k) * ps + size_t(row) * C;
-    const uint32_t* pb = planes + (size_t(r) + k) * ps + size_t(row) * C;
+    const uint32_t* pb = bplanes + size_t(k) * ps + size_t(row) * C;
     const uint2* wf = twf + size_t(k) * n2;
     const uint2* wi = twi + size_t(k) * n2;
     for (uint32_t t = threadIdx.x; t < C; t += blockDim.x) {
@@ -464,8 +469,8 @@ template <int ROWS>
 __global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restrict__ planes,
                                                                const PrimeConst* __restrict__ consts,
                                                                const uint2* __restrict__ tw, uint32_t ps,
-                                                               uint32_t n2, uint32_t C, uint32_t r) {
-    const uint32_t col = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, op = blockIdx.z;
+                                                               uint32_t n2, uint32_t C, uint32_t r, uint32_t op0) {
+    const uint32_t col = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, op = op0 + blockIdx.z;
     const uint32_t p = consts[k].p;
     uint32_t* data = planes + (size_t(op) * r + k) * ps + col;
     const uint2* w = tw + size_t(k) * n2;
@@ -598,6 +603,7 @@ __device__ __forceinline__ void row_exchange(uint32_t* buf, uint32_t (&x)[8], in
 
 template <uint32_t N1>
 __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ planes,
+                                                     const uint32_t* __restrict__ bplanes,
                                                      const PrimeConst* __restrict__ consts,
                                                      const uint2* __restrict__ twf, const uint2* __restrict__ twi,
                                                      uint32_t ps, uint32_t n2, uint32_t rows, uint32_t r) {
@@ -606,7 +612,7 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
     const uint32_t row = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
     const uint32_t p = consts[k].p, pinv = consts[k].pinv;
     uint32_t* pa = planes + size_t(k) * ps + size_t(row) * kRowC;
-    const uint32_t* pb = planes + (size_t(r) + k) * ps + size_t(row) * kRowC;
+    const uint32_t* pb = bplanes + size_t(k) * ps + size_t(row) * kRowC;
     const uint2* wf = twf + size_t(k) * n2;
     const uint2* wi = twi + size_t(k) * n2;
     // Table index of the group holding row-local jj at distance d:
@@ -675,7 +681,7 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
 // 6. reconstruction: one lane per coefficient, words in [j][n_pad] layout.
 // ---------------------------------------------------------------------------
 struct ReconstructArgs {
-    uint32_t n, n_pad, limbs, t_words;
+    uint32_t n, n_pad, limbs, t_words, i_begin, i_end;
     uint32_t s_shift, k_shift;
     uint64_t mu;
 };
@@ -688,9 +694,9 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
                                                   const uint32_t* __restrict__ corr_e,
                                                   ReconstructArgs args) {
     __shared__ uint32_t tile[64][65];
-    const uint32_t lane = threadIdx.x, i0 = blockIdx.x * 64, i = i0 + lane;
+    const uint32_t lane = threadIdx.x, i0 = args.i_begin + blockIdx.x * 64, i = i0 + lane;
     const uint32_t n = args.n, ps = args.n_pad, L = args.limbs, tw = args.t_words;
-    const bool valid = i < n;
+    const bool valid = i < args.i_end;
     const uint32_t ic = valid ? i : 0;  // clamp reads of idle lanes
 
     // K = floor((E + 2^63) / 2^64), E = sum_j 2^32j words[L+1+j] + corr_e
@@ -709,14 +715,27 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     const uint32_t sw = args.s_shift / 32, sb = args.s_shift % 32;
     uint32_t top[3] = {0, 0, 0};
     {
+        // Loads are issued 8 words ahead of the carry chain (one lane owns a
+        // whole coefficient, so memory-level parallelism has to come from here).
         int64_t carry = 0;
-        for (uint32_t j = 0; j < tw; ++j) {
-            int64_t x = (j <= L ? words[size_t(j) * ps + ic] : 0) + int64_t(corr_t[j]) + carry;
-            if (j < L) x += int64_t(kq) * int64_t(neg_p[j]);
-            const uint32_t tj = uint32_t(uint64_t(x));
-            carry = x >> 32;
-            if (valid) tsc[size_t(j) * ps + ic] = tj;
-            if (j >= sw && j < sw + 3) top[j - sw] = tj;
+        for (uint32_t j0 = 0; j0 < tw; j0 += 8) {
+            int64_t d[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                const uint32_t j = j0 + u;
+                d[u] = j <= L && j < tw ? words[size_t(j) * ps + ic] : 0;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                const uint32_t j = j0 + u;
+                if (j >= tw) break;
+                int64_t x = d[u] + int64_t(corr_t[j]) + carry;
+                if (j < L) x += int64_t(kq) * int64_t(neg_p[j]);
+                const uint32_t tj = uint32_t(uint64_t(x));
+                carry = x >> 32;
+                if (valid) tsc[size_t(j) * ps + ic] = tj;
+                if (j >= sw && j < sw + 3) top[j - sw] = tj;
+            }
         }
     }
     const unsigned __int128 topv = (unsigned __int128)top[0] | ((unsigned __int128)top[1] << 32) |
@@ -730,24 +749,31 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     uint32_t bo = 0, bo2 = 0;
     for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
         const uint32_t jend = j0 + 64 < tw ? j0 + 64 : tw;
-        for (uint32_t j = j0; j < jend; ++j) {
-            const uint32_t qj = j < L ? q[j] : 0;
-            const unsigned __int128 pr = (unsigned __int128)qe * qj + pc;
-            const uint32_t x = uint32_t(pr);
-            pc = uint64_t(pr >> 32);
-            const uint32_t tj = tsc[size_t(j) * ps + ic];
-            const uint64_t d = uint64_t(tj) - x - bo;
-            const uint32_t rj = uint32_t(d);
-            bo = uint32_t(d >> 63);
-            const uint64_t d2 = uint64_t(rj) - qj - bo2;
-            bo2 = uint32_t(d2 >> 63);
-            tile[lane][j - j0] = rj;
+        for (uint32_t g = j0; g < jend; g += 8) {
+            uint32_t tv[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) tv[u] = g + u < jend ? tsc[size_t(g + u) * ps + ic] : 0;
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                const uint32_t j = g + u;
+                if (j >= jend) break;
+                const uint32_t qj = j < L ? q[j] : 0;
+                const unsigned __int128 pr = (unsigned __int128)qe * qj + pc;
+                const uint32_t x = uint32_t(pr);
+                pc = uint64_t(pr >> 32);
+                const uint64_t d = uint64_t(tv[u]) - x - bo;
+                const uint32_t rj = uint32_t(d);
+                bo = uint32_t(d >> 63);
+                const uint64_t d2 = uint64_t(rj) - qj - bo2;
+                bo2 = uint32_t(d2 >> 63);
+                tile[lane][j - j0] = rj;
+            }
         }
         __syncthreads();
         // row-wise coalesced store: c[i0+row][j0 .. jend) for j < L
         for (uint32_t row = 0; row < 64; ++row) {
             const uint32_t gi = i0 + row, j = j0 + lane;
-            if (gi < n && j < jend && j < L) c[size_t(gi) * L + j] = tile[row][lane];
+            if (gi < args.i_end && j < jend && j < L) c[size_t(gi) * L + j] = tile[row][lane];
         }
         __syncthreads();
     }
@@ -782,62 +808,66 @@ constexpr size_t kGemmLds = 4 * size_t(kTileBytes);  // two stages of (A, B)
 
 size_t packed_operand_bytes(const Shape& s) { return size_t(2) * s.n_pad * s.conv_k; }
 
-void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, hipStream_t stream) {
+void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, uint32_t tile0,
+                 uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream) {
     const uint32_t ksteps = s.conv_k / kTK, ntiles = s.n_pad / kTile;
-    const size_t chunks = size_t(2) * ntiles * ksteps * kTile * 4;
+    const size_t chunks = size_t(nops) * tiles * ksteps * kTile * 4;
     dim3 grid(uint32_t((chunks + 255) / 256));
     hipLaunchKernelGGL(pack_operands, grid, dim3(256), 0, stream, a, b, reinterpret_cast<uint4*>(packed), s.n,
-                       s.limbs, ksteps, ntiles);
+                       s.limbs, ksteps, ntiles, tile0, tiles, op0, nops);
     check_launch("pack_operands");
 }
 
 void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
-                       hipStream_t stream) {
-    const uint32_t blocks = (s.conv_rows / kTile) * 2 * (s.n_pad / kTile);
+                       uint32_t tile0, uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream) {
+    const uint32_t blocks = (s.conv_rows / kTile) * nops * tiles;
     hipLaunchKernelGGL(conversion_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wconv, packed,
-                       t.consts, planes, s.n, s.n_pad, s.r, s.r_pad, s.conv_rows, s.conv_k / kTK);
+                       t.consts, planes, s.n, s.n_pad, s.r, s.r_pad, s.conv_rows, s.conv_k / kTK, tile0, tiles,
+                       op0, nops);
     check_launch("conversion_gemm");
 }
 
-void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream) {
+void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, uint32_t op0,
+                                uint32_t nops, hipStream_t stream) {
     if (s.rows < 2) return;
     const uint32_t ps = s.n_pad, r = s.r_pad, C = s.row_width;
     if (C % 256 == 0 && s.rows <= 64) {
-        dim3 grid(C / 256, s.r, 2);
+        dim3 grid(C / 256, s.r, nops);
         switch (s.rows) {
 #define PM_FWD(R)                                                                                                \
     case R:                                                                                                      \
         hipLaunchKernelGGL(ntt_columns_forward_reg<R>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_fwd, ps, \
-                           s.n2, C, r);                                                                          \
+                           s.n2, C, r, op0);                                                                     \
         break;
             PM_FWD(2) PM_FWD(4) PM_FWD(8) PM_FWD(16) PM_FWD(32) PM_FWD(64)
 #undef PM_FWD
         }
     } else {
         const uint32_t cw = column_width(s);
-        dim3 grid(C / cw, s.r, 2);
+        dim3 grid(C / cw, s.r, nops);
         hipLaunchKernelGGL(ntt_columns_forward, grid, dim3(256), size_t(s.rows) * cw * 4, stream, planes, t.consts,
-                           t.tw_fwd, ps, s.n2, C, s.rows, cw, r);
+                           t.tw_fwd, ps, s.n2, C, s.rows, cw, r, op0);
     }
     check_launch("ntt_columns_forward");
 }
 
-void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream) {
+void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, const uint32_t* bplanes,
+                             hipStream_t stream) {
     const uint32_t ps = s.n_pad, r = s.r_pad;
     dim3 grid(s.rows, s.r);
     if (s.row_width == kRowC && s.n1 <= 8) {
         switch (s.n1) {
 #define PM_ROWS(B)                                                                                              \
     case B:                                                                                                     \
-        hipLaunchKernelGGL(ntt_rows_2048<B>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_fwd, t.tw_inv, ps, \
-                           s.n2, s.rows, r);                                                                    \
+        hipLaunchKernelGGL(ntt_rows_2048<B>, grid, dim3(256), 0, stream, planes, bplanes, t.consts, t.tw_fwd,     \
+                           t.tw_inv, ps, s.n2, s.rows, r);                                                      \
         break;
             PM_ROWS(1) PM_ROWS(2) PM_ROWS(4) PM_ROWS(8)
 #undef PM_ROWS
         }
     } else {
-        hipLaunchKernelGGL(ntt_rows_product, grid, dim3(256), size_t(2) * s.row_width * 4, stream, planes, t.consts,
-                           t.tw_fwd, t.tw_inv, ps, s.n1, s.n2, s.row_width, s.rows, r);
+        hipLaunchKernelGGL(ntt_rows_product, grid, dim3(256), size_t(2) * s.row_width * 4, stream, planes, bplanes,
+                           t.consts, t.tw_fwd, t.tw_inv, ps, s.n1, s.n2, s.row_width, s.rows, r);
     }
     check_launch("ntt_rows_product");
 }
@@ -880,9 +910,11 @@ void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int
 }
 
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
-                        uint32_t* c, hipStream_t stream) {
-    ReconstructArgs args{s.n, s.n_pad, s.limbs, s.t_words, s.s_shift, s.k_shift, s.mu};
-    dim3 grid((s.n + 63) / 64);
+                        uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream) {
+    if (!count) return;
+    const uint32_t i_end = i_begin + count < s.n ? i_begin + count : s.n;
+    ReconstructArgs args{s.n, s.n_pad, s.limbs, s.t_words, i_begin, i_end, s.s_shift, s.k_shift, s.mu};
+    dim3 grid((count + 63) / 64);
     hipLaunchKernelGGL(reconstruct, grid, dim3(64), 0, stream, words, t_scratch, c, t.q, t.neg_p, t.corr_t,
                        t.corr_e, args);
     check_launch("reconstruct");

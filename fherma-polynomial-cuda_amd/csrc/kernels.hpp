@@ -36,18 +36,26 @@ void configure_kernels();
 size_t packed_operand_bytes(const Shape& s);
 
 // Stage launches, all on `stream`.
-void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, hipStream_t stream);
+// Coefficient tiles [tile0, tile0 + tiles) (256 coefficients each) of operands
+// op0 .. op0+nops-1 (0 = a, 1 = b; operand b uses its own weights and offsets).
+void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, uint32_t tile0,
+                 uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream);
 void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
-                       hipStream_t stream);
-void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
-void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, hipStream_t stream);
+                       uint32_t tile0, uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream);
+void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, uint32_t op0,
+                                uint32_t nops, hipStream_t stream);
+// Reads a's planes from `planes` and b's from `bplanes` ([r_pad][n_pad] each);
+// the product lands in a's planes.
+void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, const uint32_t* bplanes,
+                             hipStream_t stream);
 // Inverse row-spanning stages of plane 0; writes v_k into the CRT operand tiles
 // [n_pad/256][r_pad/16][256][64] (signed bytes).
 void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
                                 hipStream_t stream);
 void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream);
+// Coefficients [i_begin, i_begin + count), i_begin a multiple of 64.
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
-                        uint32_t* c, hipStream_t stream);
+                        uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream);
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream);
 
 }  // namespace pm

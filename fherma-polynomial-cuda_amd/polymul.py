@@ -92,6 +92,10 @@ def load_library(path=None):
         "pm_plan_export_table": ([vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
         "pm_debug_stage": ([vp, vp, vp, ctypes.c_int, vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
         "pm_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
+        "pm_operand_create": ([vp, vp, plan_pp], ctypes.c_int),
+        "pm_multiply_operand": ([vp, vp, vp, vp], ctypes.c_int),
+        "pm_multiply_operand_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "pm_operand_destroy": ([vp], None),
         "pm_get_stage_times": ([vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
@@ -203,6 +207,31 @@ class Plan:
                                                  out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(size)))
         return out
 
+    # ---- resident operand -------------------------------------------------
+    def operand(self, b):
+        """Converts and transforms b once; returns a ResidentOperand."""
+        b, _ = _u32(b)
+        if b.shape != (self.n, self.limbs):
+            raise ValueError("operand must be [N][L]")
+        handle = ctypes.c_void_p()
+        _check(self.lib, self.lib.pm_operand_create(self.handle, b.ctypes.data_as(ctypes.c_void_p),
+                                                    ctypes.byref(handle)))
+        return ResidentOperand(self, handle)
+
+    def multiply_operand(self, a, operand):
+        a, _ = _u32(a)
+        if a.shape != (self.n, self.limbs):
+            raise ValueError("operand must be [N][L]")
+        c = np.empty((self.n, self.limbs), dtype=np.uint32)
+        _check(self.lib, self.lib.pm_multiply_operand(self.handle, a.ctypes.data_as(ctypes.c_void_p), operand.handle,
+                                                      c.ctypes.data_as(ctypes.c_void_p)))
+        return c
+
+    def multiply_operand_device(self, a_ptr, operand, c_ptr, stream=None):
+        _check(self.lib, self.lib.pm_multiply_operand_device(self.handle, ctypes.c_void_p(a_ptr), operand.handle,
+                                                             ctypes.c_void_p(c_ptr),
+                                                             ctypes.c_void_p(stream) if stream else None))
+
     def set_profiling(self, enabled=True):
         _check(self.lib, self.lib.pm_set_profiling(self.handle, 1 if enabled else 0))
 
@@ -210,3 +239,28 @@ class Plan:
         out = (ctypes.c_float * len(STAGE_TIME_NAMES))()
         _check(self.lib, self.lib.pm_get_stage_times(self.handle, out, len(STAGE_TIME_NAMES)))
         return dict(zip(STAGE_TIME_NAMES, (float(x) for x in out)))
+
+
+class ResidentOperand:
+    """A factor b whose transformed residues stay on the device (pm_operand)."""
+
+    def __init__(self, plan, handle):
+        self.plan = plan
+        self.handle = handle
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.plan.lib.pm_operand_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

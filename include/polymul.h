@@ -11,8 +11,7 @@
  * std::runtime_error exactly where the reference throws.
  *
  * Threading: a plan may be used by one thread at a time.  Every entry point
- * returns only after its device work has finished, except
- * pm_multiply_device_async, which returns once the work is queued.
+ * returns only after its device work has finished.
  */
 #ifndef POLYMUL_AMD_H
 #define POLYMUL_AMD_H
@@ -79,6 +78,19 @@ pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint3
 pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c,
                              void* stream);
 
+/* Resident operand: no counterpart in the reference (its operands always
+ * arrive as a pair).  For callers that multiply many a by one fixed b: b is
+ * converted and transformed once and its residues stay on the device
+ * (r * n_pad * 4 bytes), so each later product skips b's conversion GEMM and
+ * forward transform.  An operand is tied to the plan that created it, which
+ * must outlive it. */
+typedef struct pm_operand pm_operand;
+pm_status pm_operand_create(pm_plan* plan, const uint32_t* b, pm_operand** out);
+pm_status pm_multiply_operand(pm_plan* plan, const uint32_t* a, const pm_operand* b, uint32_t* c);
+pm_status pm_multiply_operand_device(pm_plan* plan, const uint32_t* a, const pm_operand* b, uint32_t* c,
+                                     void* stream);
+void pm_operand_destroy(pm_operand* operand);
+
 /* Message describing the most recent failure on the calling thread. */
 const char* pm_last_error(void);
 
@@ -100,7 +112,10 @@ enum {
     PM_TABLE_CORR_E = 9,       /* [4] u32 */
     PM_TABLE_RECIPROCALS = 10, /* [r] u64: floor((2^64-1)/p_k) */
     PM_TABLE_BETA = 11,        /* [r] u32: operand-b scale */
-    PM_TABLE_SHAPE = 12        /* 24 x u32: see pm_plan_shape_words */
+    PM_TABLE_SHAPE = 12        /* 24 x u32: n, width, limbs, n1, n2, log2 n2, r, r_pad, conv_k,
+                                  conv_rows, crt_words, crt_words_pad, crt_rows, crt_k, t_words,
+                                  row_width, rows, qbits, s_shift, k_shift, mu (lo, hi), trivial,
+                                  reference_walk */
 };
 /* Returns the byte size of a table through *bytes when out == NULL. */
 pm_status pm_plan_export_table(const pm_plan* plan, int table, void* out, size_t* bytes);
@@ -117,10 +132,12 @@ enum { PM_STAGE_RESIDUES = 1, PM_STAGE_PRODUCT = 2, PM_STAGE_WORDS = 3, PM_STAGE
 pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, int stage, void* out,
                          size_t* bytes);
 
-/* Kernel timing of the most recent device multiply, when enabled: fills up to
- * `count` entries of ms (0: conversion GEMM, 1: forward column NTT,
- * 2: row NTT + product + inverse row NTT, 3: inverse column NTT, 4: CRT GEMM,
- * 5: reconstruction, 6: total) measured with HIP events on the plan's stream. */
+/* Kernel timing of the most recent device multiply (pm_multiply_device or
+ * pm_multiply_operand_device), when enabled: fills up to `count` entries of ms
+ * (0: operand packing, 1: conversion GEMM, 2: forward column NTT, 3: row NTT +
+ * product + inverse row NTT, 4: inverse column NTT + CRT tiling, 5: CRT GEMM,
+ * 6: reconstruction, 7: total), measured with HIP events on the stream the
+ * kernels run on. */
 pm_status pm_set_profiling(pm_plan* plan, int enabled);
 pm_status pm_get_stage_times(const pm_plan* plan, float* ms, int count);
 

@@ -1,0 +1,68 @@
+"""Resident-operand products (b transformed once, kept on the device) must be
+bit-identical to the plain product and to the oracle, for several a against
+one b, through the host and the device entry points."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,count", [("small", 4), ("mid", 2)])
+def test_resident_operand_matches_plain_product(polymul, workloads, oracle, name, count):
+    print("[gpu-case] resident", name, flush=True)
+    n, width, limbs, q, a, b = workloads.baseline_case(name)
+    with polymul.Plan(n, width, limbs, q) as plan:
+        with plan.operand(b) as fixed:
+            for seed in range(count):
+                x, _ = workloads.operands_for(n, q, 1000 + seed)
+                got = plan.multiply_operand(x, fixed)
+                assert np.array_equal(got, plan.multiply(x, b))
+                if name == "small":
+                    assert np.array_equal(got, oracle.kronecker_gmp(x, b, q))
+
+
+def test_resident_operand_device_entry(polymul, workloads, device_memory):
+    print("[gpu-case] resident device", flush=True)
+    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        expect = plan.multiply(a, b)
+        with plan.operand(b) as fixed:
+            da = device_memory.upload(a)
+            dc = device_memory.alloc(a.nbytes)
+            try:
+                plan.multiply_operand_device(da, fixed, dc)
+                got = device_memory.download(dc, a.shape, np.uint32)
+            finally:
+                device_memory.free(da)
+                device_memory.free(dc)
+            assert np.array_equal(got, expect)
+
+
+def test_operand_of_another_plan_is_rejected(polymul, workloads):
+    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    with polymul.Plan(n, width, limbs, q) as p1, polymul.Plan(n, width, limbs, q) as p2:
+        with p1.operand(b) as fixed:
+            with pytest.raises(polymul.PolymulError) as err:
+                p2.multiply_operand(a, fixed)
+            assert err.value.status == 1
+
+
+@pytest.mark.parametrize("name", ["small", "mid"])
+def test_device_entry_matches_host_entry(polymul, workloads, oracle, device_memory, name):
+    """pm_multiply_device (operands already in HBM) gives the host entry's c."""
+    print("[gpu-case] device entry", name, flush=True)
+    n, width, limbs, q, a, b = workloads.baseline_case(name)
+    with polymul.Plan(n, width, limbs, q) as plan:
+        expect = plan.multiply(a, b)
+        da, db = device_memory.upload(a), device_memory.upload(b)
+        dc = device_memory.alloc(a.nbytes)
+        try:
+            for _ in range(2):  # the second call reuses every buffer of the first
+                plan.multiply_device(da, db, dc)
+                got = device_memory.download(dc, a.shape, np.uint32)
+                assert np.array_equal(got, expect)
+        finally:
+            for ptr in (da, db, dc):
+                device_memory.free(ptr)
+        if name == "small":
+            assert np.array_equal(expect, oracle.kronecker_gmp(a, b, q))
