@@ -97,10 +97,12 @@ __device__ __forceinline__ int64_t combine4(const v4i& v) {
 }
 
 // LDS operand tile: 256 rows of 64 bytes; the four 16-byte chunks of a row are
-// XOR-swizzled by row/4, so the 16 rows one MFMA fragment reads (one chunk
-// each) fall into 16 distinct 4-bank groups.
+// XOR-swizzled by row/2.  A 128-bit LDS read serves 8 lanes (128 bytes) per
+// pass; a fragment read gives those 8 lanes rows r..r+7 of one chunk, and with
+// the swizzle their 16-byte slots (row*4 + chunk') mod 8 are all distinct.  The
+// staging writes (4 chunks of rows r, r+1 per 8 lanes) are distinct as well.
 __device__ __forceinline__ uint32_t swz(uint32_t row, uint32_t chunk) {
-    return row * kTK + (((chunk ^ (row >> 2)) & 3u) << 4);
+    return row * kTK + (((chunk ^ (row >> 1)) & 3u) << 4);
 }
 
 // Block -> (m tile, n tile).  Workgroups are dispatched to the 8 XCDs round
@@ -139,20 +141,25 @@ struct TiledB {  // pre-tiled signed bytes: [k steps][256 rows][64]
 };
 
 // The K loop: A is a pre-tiled weight panel ([k steps][256][64]); LDS holds
-// two stages of (A, B) = 64 KB; one barrier per 64-byte K step.  Wave (wm, wn)
-// computes rows wm*128 .. +128 and coefficients wn*64 .. +64 of the tile.
+// two stages of (A, B) = 64 KB and one barrier ends each 64-byte K step.  Global
+// loads run two K steps ahead through two register staging sets, so a fetch has
+// two steps of MFMA work to land: while step k computes from LDS buffer k&1, step
+// k+2 is in flight and step k+1 (fetched during step k-1) goes into the other
+// buffer.  Wave (wm, wn) computes rows wm*128 .. +128 and coefficients
+// wn*64 .. +64 of the tile.
 template <class BLoader>
 __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel, const BLoader& bl,
                                               uint32_t ksteps, int8_t* lds, v4i (&acc)[kWaveM][kWaveN]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wm = wave >> 2, wn = wave & 3;
-    v4u ra[kLoads], rb[kLoads];
-    auto load_a = [&](uint32_t ks) {
+    v4u ra0[kLoads], rb0[kLoads], ra1[kLoads], rb1[kLoads];
+    auto load = [&](uint32_t ks, v4u (&ra)[kLoads], v4u (&rb)[kLoads]) {
         const v4u* src = reinterpret_cast<const v4u*>(a_panel + size_t(ks) * kTileBytes);
 #pragma unroll
         for (uint32_t s = 0; s < kLoads; ++s) ra[s] = src[tid + kGemmThreads * s];
+        bl.load(ks, tid, rb);
     };
-    auto store = [&](uint32_t buf) {
+    auto store = [&](uint32_t buf, const v4u (&ra)[kLoads], const v4u (&rb)[kLoads]) {
         int8_t* la = lds + buf * 2 * kTileBytes;
 #pragma unroll
         for (uint32_t s = 0; s < kLoads; ++s) {
@@ -161,18 +168,7 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
         }
         bl.store(la + kTileBytes, tid, rb);
     };
-
-    load_a(0);
-    bl.load(0, tid, rb);
-    store(0);
-    __syncthreads();
-    for (uint32_t ks = 0; ks < ksteps; ++ks) {
-        const uint32_t buf = ks & 1;
-        const bool more = ks + 1 < ksteps;
-        if (more) {
-            load_a(ks + 1);
-            bl.load(ks + 1, tid, rb);
-        }
+    auto compute = [&](uint32_t buf) {
         const int8_t* la = lds + buf * 2 * kTileBytes;
         const int8_t* lb = la + kTileBytes;
         v4i fa[kWaveM], fb[kWaveN];
@@ -187,37 +183,57 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
 #pragma unroll
             for (uint32_t ni = 0; ni < kWaveN; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
-        if (more) store(buf ^ 1);
+    };
+
+    load(0, ra0, rb0);
+    if (ksteps > 1) load(1, ra1, rb1);
+    store(0, ra0, rb0);
+    __syncthreads();
+    uint32_t ks = 0;
+    for (; ks + 2 <= ksteps; ks += 2) {
+        // even step ks (buffer 0): set 0 is free, fetch ks+2; set 1 (step ks+1) -> buffer 1
+        if (ks + 2 < ksteps) load(ks + 2, ra0, rb0);
+        compute(0);
+        store(1, ra1, rb1);
+        __syncthreads();
+        // odd step ks+1 (buffer 1): set 1 is free, fetch ks+3; set 0 (step ks+2) -> buffer 0
+        if (ks + 3 < ksteps) load(ks + 3, ra1, rb1);
+        compute(1);
+        if (ks + 2 < ksteps) store(0, ra0, rb0);
         __syncthreads();
     }
+    if (ks < ksteps) compute(0);  // odd step count: the last step sits in buffer 0
 }
 
 // ---------------------------------------------------------------------------
 // 0. operand packing: [N][L] u32 -> [2][n_pad/256][conv_k/64][256][64] bytes,
-//    each byte XOR 0x80 (u -> u-128 as int8); padding bytes are zero values.
+//    each byte XOR 0x80 (u -> u-128 as int8); padding words (beyond L, or rows
+//    beyond N) hold the value 0.  One wave per coefficient row: lane l moves
+//    words l, l+64, ... of the row, so every load reads 256 contiguous bytes and
+//    each 128-byte line of the input is consumed by one wave (a block holds 4
+//    consecutive rows, so only lines at block edges are shared).
 // ---------------------------------------------------------------------------
-// Processes coefficient tiles [tile0, tile0 + tiles) of both operands.
+// Processes coefficient tiles [tile0, tile0 + tiles) of operands op0 .. op0+nops-1.
 __global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict__ a,
                                                      const uint32_t* __restrict__ b,
-                                                     uint4* __restrict__ packed, uint32_t n, uint32_t limbs,
+                                                     uint32_t* __restrict__ packed, uint32_t n, uint32_t limbs,
                                                      uint32_t ksteps, uint32_t ntiles, uint32_t tile0,
                                                      uint32_t tiles, uint32_t op0, uint32_t nops) {
-    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const size_t per_op = size_t(tiles) * ksteps * kTile * 4;
-    if (idx >= nops * per_op) return;
-    const uint32_t op = op0 + uint32_t(idx / per_op);
-    size_t rest = idx % per_op;
-    const uint32_t inner = uint32_t(rest & 1023), chunk = inner & 3, row = inner >> 2;
-    rest >>= 10;
-    const uint32_t ks = uint32_t(rest % ksteps), tn = tile0 + uint32_t(rest / ksteps);
-    const uint32_t coef = tn * kTile + row, w0 = ks * 16 + chunk * 4;
-    const uint32_t* src = (op ? b : a) + size_t(coef) * limbs;
-    uint32_t v[4];
-#pragma unroll
-    for (uint32_t c = 0; c < 4; ++c) v[c] = (coef < n && w0 + c < limbs) ? src[w0 + c] : 0u;
-    const uint32_t flip = 0x80808080u;
-    packed[((size_t(op) * ntiles + tn) * ksteps + ks) * 1024 + inner] =
-        make_uint4(v[0] ^ flip, v[1] ^ flip, v[2] ^ flip, v[3] ^ flip);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t per_op = tiles * kTile;
+    const uint32_t gr = blockIdx.x * 4 + (threadIdx.x >> 6);  // row over all operands of the launch
+    if (gr >= nops * per_op) return;
+    const uint32_t op = op0 + gr / per_op, rr = gr % per_op;
+    const uint32_t tn = tile0 + rr / kTile, row = rr % kTile, coef = tn * kTile + row;
+    const bool live = coef < n;
+    const uint32_t* src = (op ? b : a) + size_t(live ? coef : 0) * limbs;
+    uint32_t* dst = packed + (size_t(op) * ntiles + tn) * ksteps * (kTileBytes / 4) + row * (kTK / 4);
+    const uint32_t words = ksteps * (kTK / 4);
+#pragma unroll 4
+    for (uint32_t w = lane; w < words; w += 64) {
+        const uint32_t v = (live && w < limbs) ? src[w] : 0u;
+        dst[size_t(w / (kTK / 4)) * (kTileBytes / 4) + w % (kTK / 4)] = v ^ 0x80808080u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -592,6 +608,10 @@ __device__ __forceinline__ void row_inverse_group(uint32_t (&x)[8], int layout, 
     }
 }
 
+// Layouts C and D put up to 8 lanes of a wave on one LDS bank; an XOR swizzle
+// that removes those conflicts measured slower (its per-element addressing cost
+// more than the conflicts), so the exchange keeps plain indices, which the
+// compiler folds into immediate offsets.
 __device__ __forceinline__ void row_exchange(uint32_t* buf, uint32_t (&x)[8], int from, int to, uint32_t t) {
 #pragma unroll
     for (int e = 0; e < 8; ++e) buf[layout_index(from, t, e)] = x[e];
@@ -811,9 +831,9 @@ size_t packed_operand_bytes(const Shape& s) { return size_t(2) * s.n_pad * s.con
 void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, uint32_t tile0,
                  uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream) {
     const uint32_t ksteps = s.conv_k / kTK, ntiles = s.n_pad / kTile;
-    const size_t chunks = size_t(nops) * tiles * ksteps * kTile * 4;
-    dim3 grid(uint32_t((chunks + 255) / 256));
-    hipLaunchKernelGGL(pack_operands, grid, dim3(256), 0, stream, a, b, reinterpret_cast<uint4*>(packed), s.n,
+    const size_t rows = size_t(nops) * tiles * kTile;  // one wave per row, 4 rows per block
+    dim3 grid(uint32_t((rows + 3) / 4));
+    hipLaunchKernelGGL(pack_operands, grid, dim3(256), 0, stream, a, b, reinterpret_cast<uint32_t*>(packed), s.n,
                        s.limbs, ksteps, ntiles, tile0, tiles, op0, nops);
     check_launch("pack_operands");
 }
