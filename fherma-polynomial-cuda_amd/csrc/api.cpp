@@ -63,6 +63,7 @@ struct pm_plan {
     hipEvent_t in_ready[kMaxChunks] = {}, out_ready[kMaxChunks] = {}, reconstructed[kMaxChunks] = {};
     std::unique_ptr<pm::CopyPool> pool;
     uint64_t device_bytes = 0;
+    uint32_t* dev_beta = nullptr;  // device-built beta (Tables::kDevice), for table export
     bool profiling = false;
     hipEvent_t events[kEvents] = {};
     float stage_ms[kStages] = {};
@@ -207,8 +208,10 @@ struct pm_plan {
         io_c = alloc<uint32_t>(nl);
         if (s.trivial) return;
         pm::configure_kernels();
-        tables.consts = upload(host.consts);
-        {
+        if (host.tables == pm::Tables::kDevice) {
+            build_device_tables();
+        } else {
+            tables.consts = upload(host.consts);
             // both operands' weight matrices, each in 256x64 tiles
             const size_t one = size_t(s.conv_rows) * s.conv_k;
             std::vector<int8_t> tiled;
@@ -219,10 +222,10 @@ struct pm_plan {
                 tiled.insert(tiled.end(), t.begin(), t.end());
             }
             tables.wconv = upload(tiled);
+            tables.tw_fwd = reinterpret_cast<const uint2*>(upload(host.tw_fwd));
+            tables.tw_inv = reinterpret_cast<const uint2*>(upload(host.tw_inv));
+            tables.wcrt = upload(pm::tile_256x64(host.wcrt, s.crt_rows, s.crt_k));
         }
-        tables.tw_fwd = reinterpret_cast<const uint2*>(upload(host.tw_fwd));
-        tables.tw_inv = reinterpret_cast<const uint2*>(upload(host.tw_inv));
-        tables.wcrt = upload(pm::tile_256x64(host.wcrt, s.crt_rows, s.crt_k));
         tables.q = upload(host.q);
         tables.neg_p = upload(host.neg_p_mod_q);
         tables.corr_t = upload(host.corr_t);
@@ -239,6 +242,57 @@ struct pm_plan {
         // pinned staging, streams and copy workers of the host entry points are
         // set up here, with the rest of the plan, not inside the first product
         make_staging();
+    }
+
+    // Prime-set tables and CRT weights built by kernels from the O(r) and O(rL)
+    // host results (psi, reciprocals, M_k): no O(r^2) or O(r*N) host loops and no
+    // upload of the large tables.
+    void build_device_tables() {
+        const pm::Shape& s = host.shape;
+        pm::TableBuild tb;
+        tb.consts = upload(host.consts);
+        tb.psi = upload(host.psi);
+        tb.rcp = upload(host.reciprocal);
+        tb.m_mod_q = upload(host.m_mod_q);
+        tb.beta = dev_beta = alloc<uint32_t>(s.r);
+        tb.wconv = alloc<int8_t>(size_t(2) * s.conv_rows * s.conv_k);
+        tb.tw_fwd = alloc<uint32_t>(size_t(s.r) * s.n2 * 2);
+        tb.tw_inv = alloc<uint32_t>(size_t(s.r) * s.n2 * 2);
+        tb.wcrt = alloc<int8_t>(size_t(s.crt_rows) * s.crt_k);
+        pm::launch_build_tables(s, tb, stream);
+        hip_check(hipStreamSynchronize(stream), "build tables");
+        tables.consts = tb.consts;
+        tables.wconv = tb.wconv;
+        tables.tw_fwd = reinterpret_cast<const uint2*>(tb.tw_fwd);
+        tables.tw_inv = reinterpret_cast<const uint2*>(tb.tw_inv);
+        tables.wcrt = tb.wcrt;
+    }
+
+    // Host copies of device-built tables, in the host layouts of plan.cpp (for
+    // table export); fetched once.
+    void fetch_device_tables() {
+        const pm::Shape& s = host.shape;
+        if (host.tables != pm::Tables::kDevice || !device || s.trivial || !host.wconv.empty()) return;
+        hip_check(hipSetDevice(device_id), "hipSetDevice");
+        auto get = [&](auto& vec, const void* src, size_t count) {
+            vec.resize(count);
+            hip_check(hipMemcpy(vec.data(), src, count * sizeof(vec[0]), hipMemcpyDeviceToHost), "fetch tables");
+        };
+        get(host.consts, tables.consts, s.r);
+        get(host.beta, dev_beta, s.r);
+        get(host.tw_fwd, tables.tw_fwd, size_t(s.r) * s.n2 * 2);
+        get(host.tw_inv, tables.tw_inv, size_t(s.r) * s.n2 * 2);
+        std::vector<int8_t> tiled;
+        const size_t one = size_t(s.conv_rows) * s.conv_k;
+        get(tiled, tables.wconv, 2 * one);
+        host.wconv.resize(2 * one);
+        for (int op = 0; op < 2; ++op) {
+            std::vector<int8_t> part(tiled.begin() + op * one, tiled.begin() + (op + 1) * one);
+            std::vector<int8_t> flat = pm::untile_256x64(part, s.conv_rows, s.conv_k);
+            std::copy(flat.begin(), flat.end(), host.wconv.begin() + op * one);
+        }
+        get(tiled, tables.wcrt, size_t(s.crt_rows) * s.crt_k);
+        host.wcrt = pm::untile_256x64(tiled, s.crt_rows, s.crt_k);
     }
 
     // The b residues inside the work planes ([r_pad][n_pad] after a's).
@@ -350,7 +404,7 @@ pm_status create(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, 
         if (!out) throw std::invalid_argument("null output pointer");
         *out = nullptr;
         auto plan = std::make_unique<pm_plan>();
-        plan->host = pm::build_plan(n, width, limbs, q);
+        plan->host = pm::build_plan(n, width, limbs, q, device ? pm::Tables::kDevice : pm::Tables::kHost);
         if (device) plan->make_device();
         *out = plan.release();
     });
@@ -493,6 +547,8 @@ pm_status pm_plan_get_primes(const pm_plan* plan, uint32_t* primes, size_t capac
 pm_status pm_plan_export_table(const pm_plan* plan, int table, void* out, size_t* bytes) {
     return guarded([&] {
         if (!plan || !bytes) throw std::invalid_argument("null argument");
+        // device-built tables are fetched (and untiled) on first export
+        const_cast<pm_plan*>(plan)->fetch_device_tables();
         const pm::HostPlan& h = plan->host;
         switch (table) {
             case PM_TABLE_PRIME_CONSTS: copy_table(h.consts, out, bytes); break;

@@ -815,6 +815,201 @@ __global__ void fill_zero(uint32_t* __restrict__ c, size_t words) {
     if (i < words) c[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// 7. plan-time tables (host equivalents in plan.cpp, build_plan).
+// ---------------------------------------------------------------------------
+// x*y mod p for x, y < p < 2^31 with rcp = floor((2^64-1)/p): for t = xy < 2^62
+// the estimate hi64(t*rcp) lies in (t/p - 2, t/p], so two corrections finish.
+__device__ __forceinline__ uint32_t mulmod_rcp(uint32_t x, uint32_t y, uint32_t p, uint64_t rcp) {
+    const uint64_t t = uint64_t(x) * y;
+    uint64_t r = t - __umul64hi(t, rcp) * p;
+    if (r >= p) r -= p;
+    if (r >= p) r -= p;
+    return uint32_t(r);
+}
+
+__device__ __forceinline__ uint32_t powmod_rcp(uint32_t b, uint32_t e, uint32_t p, uint64_t rcp) {
+    uint32_t r = 1;
+    for (; e; e >>= 1) {
+        if (e & 1) r = mulmod_rcp(r, b, p, rcp);
+        b = mulmod_rcp(b, b, p, rcp);
+    }
+    return r;
+}
+
+// Shoup quotient floor(w * 2^32 / p) for w < p, by the same estimate.
+__device__ __forceinline__ uint32_t shoup_rcp(uint32_t w, uint32_t p, uint64_t rcp) {
+    const uint64_t t = uint64_t(w) << 32;
+    uint64_t q = __umul64hi(t, rcp), r = t - q * p;
+    if (r >= p) { r -= p; ++q; }
+    if (r >= p) ++q;
+    return uint32_t(q);
+}
+
+// beta_k = n2^-1 * 2^32 * P_k^-1 mod p_k, with P_k mod p_k = prod_{j != k} p_j
+// formed by one block per prime (strided products, then a tree).
+__global__ __launch_bounds__(256) void table_beta(const PrimeConst* __restrict__ consts,
+                                                  const uint64_t* __restrict__ rcp, uint32_t* __restrict__ beta,
+                                                  uint32_t r, uint32_t n2) {
+    __shared__ uint32_t part[256];
+    const uint32_t k = blockIdx.x, t = threadIdx.x;
+    const uint32_t p = consts[k].p;
+    const uint64_t rc = rcp[k];
+    uint32_t prod = 1;
+    for (uint32_t j = t; j < r; j += 256)
+        if (j != k) prod = mulmod_rcp(prod, consts[j].p % p, p, rc);
+    part[t] = prod;
+    __syncthreads();
+    for (uint32_t w = 128; w; w >>= 1) {
+        if (t < w) part[t] = mulmod_rcp(part[t], part[t + w], p, rc);
+        __syncthreads();
+    }
+    if (t == 0) {
+        const uint32_t radix = uint32_t((uint64_t(1) << 32) % p);
+        const uint32_t n2_inv = powmod_rcp(n2 % p, p - 2, p, rc);
+        const uint32_t pk_inv = powmod_rcp(part[0], p - 2, p, rc);
+        beta[k] = mulmod_rcp(mulmod_rcp(n2_inv, radix, p, rc), pk_inv, p, rc);
+    }
+}
+
+// tw[k][j] = (psi^e, Shoup quotient) with e = bitrev(j) over log2(n2) bits; the
+// inverse table uses psi^-1.  grid (column blocks, r).
+__global__ __launch_bounds__(256) void table_twiddles(const PrimeConst* __restrict__ consts,
+                                                      const uint64_t* __restrict__ rcp,
+                                                      const uint32_t* __restrict__ psi, uint32_t* __restrict__ fwd,
+                                                      uint32_t* __restrict__ inv, uint32_t n2, uint32_t log_n2) {
+    __shared__ uint32_t pf[32], pi[32];  // psi^(2^i), psi^-(2^i)
+    const uint32_t k = blockIdx.y;
+    const uint32_t p = consts[k].p;
+    const uint64_t rc = rcp[k];
+    if (threadIdx.x == 0) {
+        uint32_t a = psi[2 * k], b = psi[2 * k + 1];
+        for (uint32_t i = 0; i < log_n2; ++i) {
+            pf[i] = a;
+            pi[i] = b;
+            a = mulmod_rcp(a, a, p, rc);
+            b = mulmod_rcp(b, b, p, rc);
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n2; j += gridDim.x * 256) {
+        const uint32_t e = log_n2 ? __brev(j) >> (32 - log_n2) : 0;
+        uint32_t w = 1, wi = 1;
+        for (uint32_t i = 0; i < log_n2; ++i)
+            if ((e >> i) & 1) {
+                w = mulmod_rcp(w, pf[i], p, rc);
+                wi = mulmod_rcp(wi, pi[i], p, rc);
+            }
+        const size_t at = (size_t(k) * n2 + j) * 2;
+        *reinterpret_cast<uint2*>(fwd + at) = make_uint2(w, shoup_rcp(w, p, rc));
+        *reinterpret_cast<uint2*>(inv + at) = make_uint2(wi, shoup_rcp(wi, p, rc));
+    }
+}
+
+// Conversion weights of operand op for prime k: columns j < 4L hold
+// lambda * 256^j mod p (lambda = 2^32 for a, 2^32 * beta_k for b), centered and
+// split into four signed digits (rows 4k+d); columns j >= 4L are zero.  The
+// block also forms off = 128 * sum_j centered_j mod p.  Each thread writes four
+// consecutive columns of a row as one word into the tiled matrix (pre-zeroed).
+// grid (r, 2).
+__global__ __launch_bounds__(256) void table_conversion(PrimeConst* __restrict__ consts,
+                                                        const uint64_t* __restrict__ rcp,
+                                                        const uint32_t* __restrict__ beta,
+                                                        int8_t* __restrict__ wtiles, uint32_t conv_rows,
+                                                        uint32_t conv_k, uint32_t limbs) {
+    __shared__ int64_t part[256];
+    const uint32_t k = blockIdx.x, op = blockIdx.y, t = threadIdx.x;
+    const uint32_t p = consts[k].p;
+    const uint64_t rc = rcp[k];
+    const uint32_t radix = uint32_t((uint64_t(1) << 32) % p);
+    const uint32_t lambda = op ? mulmod_rcp(radix, beta[k], p, rc) : radix;
+    const uint32_t ksteps = conv_k / kTK, num_m = conv_rows / kTile, kbytes = 4 * limbs;
+    const uint32_t step = powmod_rcp(256, 4 * 256, p, rc);  // 256^(columns per block pass)
+    uint32_t cur = mulmod_rcp(lambda, powmod_rcp(256, 4 * t, p, rc), p, rc);
+    int64_t sum = 0;
+    for (uint32_t j0 = 4 * t; j0 < conv_k; j0 += 4 * 256) {
+        uint32_t word[4] = {0, 0, 0, 0};
+        uint32_t w = cur;
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (j0 + i < kbytes) {
+                int64_t x = w > p / 2 ? int64_t(w) - p : int64_t(w);
+                sum += x;
+                for (uint32_t d = 0; d < 4; ++d) {
+                    const int64_t e = ((x + 128) & 255) - 128;
+                    word[d] |= uint32_t(uint8_t(int8_t(e))) << (8 * i);
+                    x = (x - e) >> 8;
+                }
+            }
+            w = mulmod_rcp(w, 256, p, rc);
+        }
+        for (uint32_t d = 0; d < 4; ++d) {
+            if (!word[d]) continue;
+            const uint32_t row = 4 * k + d;
+            const size_t at = ((size_t(op * num_m + row / kTile) * ksteps + j0 / kTK) * kTile + row % kTile) * kTK +
+                              j0 % kTK;
+            *reinterpret_cast<uint32_t*>(wtiles + at) = word[d];
+        }
+        cur = mulmod_rcp(cur, step, p, rc);
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t w = 128; w; w >>= 1) {
+        if (t < w) part[t] += part[t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        int64_t off = (128 * part[0]) % int64_t(p);
+        if (off < 0) off += p;
+        if (op) consts[k].off_b = uint32_t(off);
+        else consts[k].off_a = uint32_t(off);
+    }
+}
+
+// CRT weights of prime k (one thread each): column 4k+d of row c holds signed
+// digit c-d of M_k (rows 0 .. 4L+3) and, from row 4(L+1) on, signed digit
+// c-4(L+1)-d of rcp_k.  Row c's four columns are one word whose byte d is digit
+// c-d: a sliding window over the digit stream.  Matrix pre-zeroed.
+__global__ __launch_bounds__(64) void table_crt(const uint32_t* __restrict__ m_mod_q,
+                                                const uint64_t* __restrict__ rcp, int8_t* __restrict__ wtiles,
+                                                uint32_t r, uint32_t limbs, uint32_t crt_k) {
+    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= r) return;
+    const uint32_t ksteps = crt_k / kTK, col = 4 * k;
+    auto put = [&](uint32_t row, uint32_t word) {
+        if (!word) return;
+        const size_t at = ((size_t(row / kTile) * ksteps + col / kTK) * kTile + row % kTile) * kTK + col % kTK;
+        *reinterpret_cast<uint32_t*>(wtiles + at) = word;
+    };
+    const uint32_t* m = m_mod_q + size_t(k) * limbs;
+    const uint32_t md = 4 * limbs + 1;
+    uint32_t window = 0;
+    int carry = 0;
+    for (uint32_t c = 0; c < md + 3; ++c) {
+        int digit = 0;
+        if (c < md) {
+            const int x = (c / 4 < limbs ? int((m[c / 4] >> (8 * (c % 4))) & 255u) : 0) + carry;
+            carry = x >= 128;
+            digit = carry ? x - 256 : x;
+        }
+        window = (window << 8) | uint8_t(int8_t(digit));
+        put(c, window);
+    }
+    const uint64_t rv = rcp[k];
+    const uint32_t base = 4 * (limbs + 1);
+    window = 0;
+    carry = 0;
+    for (uint32_t c = 0; c < 6 + 3; ++c) {
+        int digit = 0;
+        if (c < 6) {
+            const int x = int((rv >> (8 * c)) & 255u) + carry;
+            carry = x >= 128;
+            digit = carry ? x - 256 : x;
+        }
+        window = (window << 8) | uint8_t(int8_t(digit));
+        put(base + c, window);
+    }
+}
+
 uint32_t column_width(const Shape& s) {
     uint32_t cw = 8192 / s.rows;
     if (cw > s.row_width) cw = s.row_width;
@@ -945,6 +1140,24 @@ void launch_zero(uint32_t* c, size_t words, hipStream_t stream) {
     dim3 grid(uint32_t((words + 255) / 256));
     hipLaunchKernelGGL(fill_zero, grid, dim3(256), 0, stream, c, words);
     check_launch("fill_zero");
+}
+
+void launch_build_tables(const Shape& s, const TableBuild& t, hipStream_t stream) {
+    const uint32_t r = s.r;
+    launch_zero(reinterpret_cast<uint32_t*>(t.wconv), size_t(2) * s.conv_rows * s.conv_k / 4, stream);
+    launch_zero(reinterpret_cast<uint32_t*>(t.wcrt), size_t(s.crt_rows) * s.crt_k / 4, stream);
+    hipLaunchKernelGGL(table_beta, dim3(r), dim3(256), 0, stream, t.consts, t.rcp, t.beta, r, s.n2);
+    check_launch("table_beta");
+    const uint32_t cols = (s.n2 + 255) / 256;
+    hipLaunchKernelGGL(table_twiddles, dim3(cols < 64 ? cols : 64, r), dim3(256), 0, stream, t.consts, t.rcp, t.psi,
+                       t.tw_fwd, t.tw_inv, s.n2, s.log_n2);
+    check_launch("table_twiddles");
+    hipLaunchKernelGGL(table_conversion, dim3(r, 2), dim3(256), 0, stream, t.consts, t.rcp, t.beta, t.wconv,
+                       s.conv_rows, s.conv_k, s.limbs);
+    check_launch("table_conversion");
+    hipLaunchKernelGGL(table_crt, dim3((r + 63) / 64), dim3(64), 0, stream, t.m_mod_q, t.rcp, t.wcrt, r, s.limbs,
+                       s.crt_k);
+    check_launch("table_crt");
 }
 
 void configure_kernels() {

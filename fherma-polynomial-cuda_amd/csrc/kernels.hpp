@@ -58,4 +58,22 @@ void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* wo
                         uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream);
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream);
 
+// Plan-time table construction on the device (plans built with Tables::kDevice):
+// beta_k, the merged-twist twiddles, both conversion weight matrices and their
+// signed-byte offsets (written into consts[k].off_a / off_b), and the CRT weight
+// matrix, each in the layout the product kernels read.  Bit-identical to the
+// host construction in plan.cpp.
+struct TableBuild {
+    PrimeConst* consts = nullptr;        // [r]: p, pinv set; offsets filled in
+    const uint32_t* psi = nullptr;       // [r][2]: psi, psi^-1
+    const uint64_t* rcp = nullptr;       // [r]: floor((2^64-1)/p_k)
+    const uint32_t* m_mod_q = nullptr;   // [r][L]: M_k = P_k mod q
+    uint32_t* beta = nullptr;            // [r] out
+    int8_t* wconv = nullptr;             // [2] x tile_256x64([conv_rows][conv_k]) out
+    uint32_t* tw_fwd = nullptr;          // [r][n2][2] out
+    uint32_t* tw_inv = nullptr;          // [r][n2][2] out
+    int8_t* wcrt = nullptr;              // tile_256x64([crt_rows][crt_k]) out
+};
+void launch_build_tables(const Shape& s, const TableBuild& t, hipStream_t stream);
+
 }  // namespace pm

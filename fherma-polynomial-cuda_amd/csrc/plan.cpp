@@ -17,8 +17,11 @@
 //  Barrett     mu = floor(2^(h+63)/q) for the final reduction of T mod q.
 #include "plan.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <string>
+#include <thread>
 
 namespace pm {
 namespace {
@@ -122,6 +125,19 @@ std::vector<int8_t> tile_256x64(const std::vector<int8_t>& src, uint32_t rows, u
     return out;
 }
 
+std::vector<int8_t> untile_256x64(const std::vector<int8_t>& tiled, uint32_t rows, uint32_t cols) {
+    if (rows % 256 || cols % 64 || tiled.size() != size_t(rows) * cols)
+        throw std::logic_error("plan: tiling shape mismatch");
+    std::vector<int8_t> out(tiled.size());
+    const uint32_t ksteps = cols / 64;
+    for (uint32_t r = 0; r < rows; ++r)
+        for (uint32_t ks = 0; ks < ksteps; ++ks) {
+            const size_t src = ((size_t(r / 256) * ksteps + ks) * 256 + (r % 256)) * 64;
+            std::memcpy(out.data() + size_t(r) * cols + size_t(ks) * 64, tiled.data() + src, 64);
+        }
+    return out;
+}
+
 bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes) {
     primes->clear();
     Words product{1};
@@ -137,13 +153,15 @@ bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes
     return true;
 }
 
-HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words) {
+HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words, Tables tables) {
     if (!n || (n & (n - 1)) || n >= (1u << 29))
         throw std::invalid_argument("invalid point: N must be a power of two below 2^29");
     if (!width || !limbs || limbs != (width + 31) / 32)
         throw std::invalid_argument("invalid point: L must equal ceil(W/32)");
     if (!q_words) throw std::invalid_argument("invalid point: missing modulus");
     HostPlan plan;
+    plan.tables = tables;
+    const bool host_tables = tables == Tables::kHost;
     Shape& s = plan.shape;
     s.n = n;
     s.width = width;
@@ -193,17 +211,30 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
     if (s.rows > 8192) throw std::invalid_argument("invalid point: transform exceeds the supported size (N <= 2^24)");
 
     const uint32_t r = s.r, L = limbs, n2 = s.n2;
-    plan.consts.resize(r);
-    plan.beta.resize(r);
-    plan.wconv.assign(size_t(2) * s.conv_rows * s.conv_k, 0);
-    plan.tw_fwd.assign(size_t(r) * n2 * 2, 0);
-    plan.tw_inv.assign(size_t(r) * n2 * 2, 0);
+    plan.consts.assign(r, PrimeConst{0, 0, 0, 0});
+    plan.psi.resize(size_t(r) * 2);
+    if (host_tables) {
+        plan.beta.resize(r);
+        plan.wconv.assign(size_t(2) * s.conv_rows * s.conv_k, 0);
+        plan.tw_fwd.assign(size_t(r) * n2 * 2, 0);
+        plan.tw_inv.assign(size_t(r) * n2 * 2, 0);
+    }
 
     for (uint32_t k = 0; k < r; ++k) {
         const uint32_t p = plan.primes[k];
         PrimeConst& c = plan.consts[k];
         c.p = p;
         c.pinv = newton_inverse(p);
+        // psi: a primitive 2*n2-th root of unity (psi^n2 == -1), as solve.cu:951.
+        uint32_t psi = 0;
+        for (uint32_t base = 2; !psi; ++base) {
+            uint32_t root = pow_mod(base, (p - 1) / (2 * uint64_t(n2)), p);
+            if (pow_mod(root, n2, p) == p - 1) psi = root;
+        }
+        const uint32_t psi_inv = inv_mod(psi, p);
+        plan.psi[2 * size_t(k)] = psi;
+        plan.psi[2 * size_t(k) + 1] = psi_inv;
+        if (!host_tables) continue;
         // P_k mod p_k = prod_{j != k} p_j mod p_k (for P_k^-1 mod p_k)
         uint64_t pk = 1;
         for (uint32_t j = 0; j < r; ++j)
@@ -233,13 +264,6 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
             if (off < 0) off += p;
             (op ? c.off_b : c.off_a) = uint32_t(off);
         }
-        // psi: a primitive 2*n2-th root of unity (psi^n2 == -1), as solve.cu:951.
-        uint32_t psi = 0;
-        for (uint32_t base = 2; !psi; ++base) {
-            uint32_t root = pow_mod(base, (p - 1) / (2 * uint64_t(n2)), p);
-            if (pow_mod(root, n2, p) == p - 1) psi = root;
-        }
-        const uint32_t psi_inv = inv_mod(psi, p);
         std::vector<uint32_t> pw(n2), pwi(n2);
         pw[0] = pwi[0] = 1;
         for (uint32_t e = 1; e < n2; ++e) {
@@ -267,30 +291,58 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
 
     plan.m_mod_q.assign(size_t(r) * L, 0);
     plan.reciprocal.resize(r);
+    // M_k for every prime: independent O(L) work per prime, spread over a few
+    // host threads (each writes its own rows of m_mod_q).
+    auto m_range = [&](uint32_t k0, uint32_t k1, std::string* error) {
+        try {
+            for (uint32_t k = k0; k < k1; ++k) {
+                const uint32_t p = plan.primes[k];
+                Words m;
+                const uint32_t q_mod = mod_word(q, p);
+                if (q_mod != 0) {
+                    // M_k = (Pq + q*t)/p_k with t = -Pq * q^-1 mod p_k: exact, and
+                    // congruent to Pq * p_k^-1 = P_k (mod q).  O(L) per prime.
+                    const uint32_t pq_mod = mod_word(pq, p);
+                    const uint32_t t = (p - mul_mod(pq_mod, inv_mod(q_mod, p), p)) % p;
+                    uint32_t rem = 0;
+                    m = divmod_word(add(pq, mul_word(q, t)), p, &rem);
+                    if (rem) throw std::logic_error("plan: CRT weight division not exact");
+                    if (compare(m, q) >= 0) m = sub(m, q);
+                } else {
+                    // p_k divides q: prod_{j != k} p_j mod q directly.
+                    m = Words{1};
+                    for (uint32_t j = 0; j < r; ++j)
+                        if (j != k) m = mul_word_mod(m, plan.primes[j], q);
+                }
+                for (size_t i = 0; i < m.size() && i < L; ++i) plan.m_mod_q[size_t(k) * L + i] = m[i];
+            }
+        } catch (const std::exception& e) {
+            *error = e.what();
+        }
+    };
+    {
+        const uint64_t work = uint64_t(r) * L;
+        unsigned threads = std::thread::hardware_concurrency();
+        threads = threads ? std::min(threads, 16u) : 1u;
+        if (work < (uint64_t(1) << 18)) threads = 1;
+        threads = std::min<unsigned>(threads, r);
+        std::vector<std::string> errors(threads);
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < threads; ++t)
+            pool.emplace_back(m_range, uint32_t(uint64_t(r) * t / threads), uint32_t(uint64_t(r) * (t + 1) / threads),
+                              &errors[t]);
+        m_range(0, uint32_t(uint64_t(r) / threads), &errors[0]);
+        for (auto& th : pool) th.join();
+        for (const auto& e : errors)
+            if (!e.empty()) throw std::logic_error(e);
+    }
     Words sum_m{0};
     unsigned __int128 sum_rcp = 0;
     for (uint32_t k = 0; k < r; ++k) {
-        const uint32_t p = plan.primes[k];
-        Words m;
-        const uint32_t q_mod = mod_word(q, p);
-        if (q_mod != 0) {
-            // M_k = (Pq + q*t)/p_k with t = -Pq * q^-1 mod p_k: exact, and
-            // congruent to Pq * p_k^-1 = P_k (mod q).  O(L) per prime.
-            const uint32_t pq_mod = mod_word(pq, p);
-            const uint32_t t = (p - mul_mod(pq_mod, inv_mod(q_mod, p), p)) % p;
-            uint32_t rem = 0;
-            m = divmod_word(add(pq, mul_word(q, t)), p, &rem);
-            if (rem) throw std::logic_error("plan: CRT weight division not exact");
-            if (compare(m, q) >= 0) m = sub(m, q);
-        } else {
-            // p_k divides q: prod_{j != k} p_j mod q directly.
-            m = Words{1};
-            for (uint32_t j = 0; j < r; ++j)
-                if (j != k) m = mul_word_mod(m, plan.primes[j], q);
-        }
-        for (size_t i = 0; i < m.size() && i < L; ++i) plan.m_mod_q[size_t(k) * L + i] = m[i];
+        Words m(plan.m_mod_q.begin() + size_t(k) * L, plan.m_mod_q.begin() + size_t(k + 1) * L);
+        trim(m);
         sum_m = add(sum_m, m);
-        plan.reciprocal[k] = UINT64_MAX / p;
+        plan.reciprocal[k] = UINT64_MAX / plan.primes[k];
         sum_rcp += plan.reciprocal[k];
     }
 
@@ -316,9 +368,9 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
     // CRT weights: row c = output byte column, col 4k+d = byte d of v_k.
     // T rows: c in [0, 4(L+1)), weight = digit_{c-d}(M_k) (4L+1 signed digits).
     // Quotient rows: c = 4(L+1) + e, weight = digit_{e-d}(rcp_k) (6 signed digits).
-    plan.wcrt.assign(size_t(s.crt_rows) * s.crt_k, 0);
     const size_t md = 4 * size_t(L) + 1;
-    for (uint32_t k = 0; k < r; ++k) {
+    if (host_tables) plan.wcrt.assign(size_t(s.crt_rows) * s.crt_k, 0);
+    for (uint32_t k = 0; k < r && host_tables; ++k) {
         std::vector<int8_t> dm = signed_digits(plan.m_mod_q.data() + size_t(k) * L, L, md);
         uint32_t rw[2] = {uint32_t(plan.reciprocal[k]), uint32_t(plan.reciprocal[k] >> 32)};
         std::vector<int8_t> dr = signed_digits(rw, 2, 6);

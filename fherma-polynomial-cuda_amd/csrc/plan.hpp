@@ -45,10 +45,18 @@ struct Shape {
     bool reference_walk = true;  // primes are exactly fherma_init's walk (n1 == 1)
 };
 
+// Which tables build_plan fills on the host.  kDevice leaves the prime-set
+// tables (wconv, tw_fwd, tw_inv, beta, conversion offsets) and the CRT weight
+// matrix wcrt empty: the device builds them from primes, psi, reciprocal and
+// m_mod_q (kernels.hpp, launch_build_tables).
+enum class Tables { kHost, kDevice };
+
 struct HostPlan {
     Shape shape;
+    Tables tables = Tables::kHost;
     std::vector<uint32_t> primes;
-    std::vector<PrimeConst> consts;
+    std::vector<PrimeConst> consts;   // off_a/off_b are 0 under Tables::kDevice
+    std::vector<uint32_t> psi;        // [r][2]  psi (primitive 2*n2-th root), psi^-1
     std::vector<int8_t> wconv;        // [2][conv_rows][conv_k]
     std::vector<uint32_t> tw_fwd;     // [r][n2][2]  (psi^bitrev(j), Shoup quotient)
     std::vector<uint32_t> tw_inv;     // [r][n2][2]  (psi^-bitrev(j), Shoup quotient)
@@ -66,7 +74,8 @@ struct HostPlan {
 // Validation of the point, as fherma_init does (solve.cu:903-904): N a power of
 // two below 2^29, L == ceil(W/32), q of exactly L limbs' storage, 1 <= q.
 // Throws std::invalid_argument / std::runtime_error with a message.
-HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words);
+HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words,
+                    Tables tables = Tables::kHost);
 
 // fherma_init's prime walk, generalized: descending p = 1 (mod step) from
 // floor((2^31-2)/step)*step + 1, primes only, while prod(p) <= bound, failing
@@ -79,5 +88,7 @@ bool is_prime_u32(uint32_t p);
 // of 256, cols a multiple of 64) as [rows/256][cols/64][256][64], so that every
 // (256-row tile, 64-byte K step) block is one contiguous 16 KB chunk.
 std::vector<int8_t> tile_256x64(const std::vector<int8_t>& src, uint32_t rows, uint32_t cols);
+// Inverse of tile_256x64.
+std::vector<int8_t> untile_256x64(const std::vector<int8_t>& tiled, uint32_t rows, uint32_t cols);
 
 }  // namespace pm
