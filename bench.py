@@ -86,6 +86,111 @@ def time_host(plan, a, b, runs):
     return times, c
 
 
+GiB = 1 << 30
+
+
+def log_uniform_int(rng, lo, hi):
+    return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+
+def make_products(workloads, count, seed, n_choices, limb_range):
+    """Independent products, each with its own seeded random odd q and uniform operands."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(n_choices[rng.integers(0, len(n_choices))])
+        limbs = log_uniform_int(rng, *limb_range)
+        prng = np.random.default_rng([seed, i])
+        q = workloads.random_modulus(limbs, prng)
+        out.append((n, limbs, q, workloads.uniform_below(q, n, prng), workloads.uniform_below(q, n, prng)))
+    return out
+
+
+def plain_products(polymul, products):
+    """One at a time, as a harness would run fherma_init + fherma_run per point:
+    a plan for the product's own q, host vectors in and out."""
+    outs, times = [], []
+    for n, limbs, q, a, b in products:
+        t0 = time.perf_counter()
+        with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+            outs.append(plan.multiply(a, b))
+        times.append((time.perf_counter() - t0) * 1000.0)
+    return outs, times
+
+
+def batched_leg(polymul, workloads, count, lanes):
+    """64 products: N in {2^12..2^16}, L log-uniform in [16, 1747] limbs, own q
+    each; device footprint capped at 60 GB; plain path vs pm_multiply_batch."""
+    products = make_products(workloads, count, 0x5EED0004, [4096, 8192, 16384, 32768, 65536], (16, 1747))
+    t0 = time.perf_counter()
+    plain, _ = plain_products(polymul, products)
+    plain_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched = polymul.multiply_batch(products, 60 * GiB, lanes=lanes)
+    batched_s = time.perf_counter() - t0
+    exact = all(np.array_equal(x, y) for x, y in zip(plain, batched))
+    words = sum(n * limbs for n, limbs, *_ in products)
+    return {"products": count, "lanes": lanes, "device_budget_gib": 60,
+            "plain_products_per_s": round(count / plain_s, 2), "batched_products_per_s": round(count / batched_s, 2),
+            "batched_vs_plain": round(plain_s / batched_s, 2), "exact_vs_plain": exact,
+            "host_link_gb": round(12 * words / 1e9, 2)}, products
+
+
+def resident_leg(polymul, workloads, count, n, limbs):
+    """64 random a times one fixed b: plain products vs b's residues kept on the device."""
+    rng = np.random.default_rng(0x5EED0005)
+    q = workloads.random_modulus(limbs, rng)
+    b = workloads.uniform_below(q, n, rng)
+    a_list = [workloads.uniform_below(q, n, rng) for _ in range(count)]
+    with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+        plan.multiply(a_list[0], b)
+        t0 = time.perf_counter()
+        plain = [plan.multiply(a, b) for a in a_list]
+        plain_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        with plan.operand(b) as fixed:
+            resident = [plan.multiply_operand(a, fixed) for a in a_list]
+        resident_s = time.perf_counter() - t0
+    exact = all(np.array_equal(x, y) for x, y in zip(plain, resident))
+    return {"products": count, "n": n, "limbs": limbs, "plain_products_per_s": round(count / plain_s, 2),
+            "resident_products_per_s": round(count / resident_s, 2),
+            "resident_vs_plain": round(plain_s / resident_s, 2), "exact_vs_plain": exact}
+
+
+def served_leg(polymul, workloads, count, lanes, rates):
+    """Open-loop Poisson arrivals at each rate; N = 2^U, U uniform in 12..16, L
+    log-uniform in [32, 1747] limbs, own q each; 200 GB admission budget.
+    Latency = submission to result (server clock); every result must equal the
+    same request run alone through the plain path."""
+    products = make_products(workloads, count, 0x5EED0006, [4096, 8192, 16384, 32768, 65536], (32, 1747))
+    alone, alone_ms = plain_products(polymul, products)
+    sweeps = []
+    for rate in rates:
+        rng = np.random.default_rng(0x5EED0007)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+        with polymul.Server(200 * GiB, lanes=lanes) as server:
+            start = time.perf_counter()
+            tickets = []
+            for at, prod in zip(arrivals, products):
+                delay = start + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append((time.perf_counter(), server.submit(*prod)))
+            results = [(sub, server.wait(t)) for sub, t in tickets]
+        finish = max(sub + times["total_ms"] / 1000.0 for sub, (_, times) in results)
+        lat = np.array([times["total_ms"] for _, (_, times) in results])
+        exact = all(np.array_equal(c, e) for (_, (c, _)), e in zip(results, alone))
+        sweeps.append({"offered_per_s": round(rate, 2), "products_per_s": round(count / (finish - start), 2),
+                       "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                       "queue_p99_ms": round(float(np.percentile([t["queue_ms"] for _, (_, t) in results], 99)), 2),
+                       "exact_vs_alone": exact})
+    return {"requests_per_rate": count, "lanes": lanes, "device_budget_gib": 200,
+            "plain_one_at_a_time_products_per_s": round(count / (sum(alone_ms) / 1000.0), 2),
+            "plain_p50_ms": round(float(np.percentile(alone_ms, 50)), 2),
+            "plain_p99_ms": round(float(np.percentile(alone_ms, 99)), 2), "sweep": sweeps}
+
+
 def cpu_baseline(q, a, b, plan_cls, sample_n):
     """The oracle's GMP Kronecker product (one core) on a bounded sample: the
     first sample_n coefficients of each operand as an N=sample_n product with
@@ -119,6 +224,10 @@ def main():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--no-secondary", action="store_true", help="skip the reference-point and e2e legs")
     parser.add_argument("--cpu-sample", type=int, default=8192)
+    parser.add_argument("--no-batch", action="store_true", help="skip the batched, resident and served legs")
+    parser.add_argument("--batch-count", type=int, default=64)
+    parser.add_argument("--serve-count", type=int, default=48)
+    parser.add_argument("--lanes", type=int, default=4)
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="also run one plain 1 GiB device-to-device copy (counter calibration)")
     args = parser.parse_args()
@@ -210,6 +319,18 @@ def main():
                                                      "p99": round(float(np.percentile(r_e2e, 99)), 3),
                                                      "min": round(min(r_e2e), 3), "runs": len(r_e2e)},
                                      "published_ms_e2e_rtx_pro_6000": 1.0712435}
+
+    if not args.no_batch:
+        # one untimed small product first: HIP and kernel-module start-up belong to neither leg
+        warm = make_products(workloads, 1, 1, [4096], (16, 17))
+        plain_products(polymul, warm)
+        batched, _ = batched_leg(polymul, workloads, args.batch_count, args.lanes)
+        result["batched"] = batched
+        result["resident_operand"] = resident_leg(polymul, workloads, 64, 65536, 55)
+        capacity = batched["batched_products_per_s"]
+        result["served"] = served_leg(polymul, workloads, args.serve_count, args.lanes,
+                                      [0.3 * capacity, 0.6 * capacity, 0.9 * capacity])
+        print("[bench] batched/served legs done", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         cpu_s, scale, exact, reference = cpu_baseline(q, a, b, polymul.Plan, args.cpu_sample)

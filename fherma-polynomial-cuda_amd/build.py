@@ -1,6 +1,6 @@
 """Builds the gfx950 library and its C++ front ends in-tree with hipcc.
 
-  libpolymul_amd.so     kernels.hip + api.cpp + plan.cpp   (C ABI, include/polymul.h)
+  libpolymul_amd.so     kernels.hip + api.cpp + serve.cpp + plan.cpp   (C ABI, include/polymul.h)
   libfherma_amd.so      fherma_shim.cpp linked to the above (include/fherma.h)
   solution              driver_main.cpp: the main.cpp-compatible case runner
 
@@ -41,7 +41,7 @@ def build(verbose=False):
     obj_dir = PKG / "_build"
     obj_dir.mkdir(exist_ok=True)
     objects = []
-    for src in ("kernels.hip", "api.cpp", "plan.cpp"):
+    for src in ("kernels.hip", "api.cpp", "serve.cpp", "plan.cpp"):
         obj = obj_dir / (src.rsplit(".", 1)[0] + ".o")
         flags = COMMON + (DEVICE if src.endswith(".hip") else [])
         lang = ["-x", "hip"] if src.endswith(".hip") else []

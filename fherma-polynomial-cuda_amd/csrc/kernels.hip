@@ -28,6 +28,7 @@
 // 256, so every GEMM tile load is in bounds and 16-byte aligned.
 #include "kernels.hpp"
 
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -1161,16 +1162,18 @@ void launch_build_tables(const Shape& s, const TableBuild& t, hipStream_t stream
 }
 
 void configure_kernels() {
-    // The GEMMs use 64 KB of dynamic LDS (two stages of A and B tiles).
-    static bool done = false;
-    if (done) return;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&conversion_gemm),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                int(kGemmLds));
-    if (e != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    done = true;
+    // The GEMMs use 64 KB of dynamic LDS (two stages of A and B tiles).  Plans
+    // may be created on several threads at once (server workers): set once.
+    static std::once_flag once;
+    static hipError_t status = hipSuccess;
+    std::call_once(once, [] {
+        status = hipFuncSetAttribute(reinterpret_cast<const void*>(&conversion_gemm),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
+        if (status == hipSuccess)
+            status = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
+    });
+    if (status != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(status));
 }
 
 }  // namespace pm

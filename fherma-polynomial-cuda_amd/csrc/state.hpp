@@ -1,0 +1,558 @@
+// Internal state of the library (not part of the C ABI): error mapping, the
+// per-worker HIP resources (Lane), the device memory arena of a server, and the
+// plan and resident-operand objects behind the opaque handles of polymul.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <condition_variable>
+#include <cstring>
+#include <iterator>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "copy_pool.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+#include "polymul.h"
+
+extern thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NoDevice : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* where) {
+    if (e != hipSuccess) throw DeviceError(std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Device allocations are rounded to 256 bytes (GEMM tiles load 16-byte vectors;
+// every buffer starts on a 256-byte boundary).
+inline size_t aligned_bytes(size_t bytes) { return ((bytes ? bytes : 1) + 255) & ~size_t(255); }
+
+// Events bracket the 7 stage launches; times are the 7 intervals plus the total.
+constexpr int kEvents = 8;
+constexpr int kStages = 8;
+
+namespace pm {
+
+// The host-vector path moves operands in and results out in up to this many
+// coefficient chunks, overlapping copies, transfers and the per-coefficient stages.
+constexpr int kMaxChunks = 8;
+
+// HIP resources of one sequence of host-vector products: a compute stream and two
+// copy streams (all default priority), the chunk events, stage-timing events,
+// pinned staging that grows to the largest product seen, and host copy workers.
+// A standalone plan owns one; a server worker owns one and lends it to each plan
+// it runs, so a stream of requests pays for none of this per request.
+class Lane {
+public:
+    explicit Lane(unsigned copy_threads) {
+        try {
+            for (hipStream_t* st : {&stream, &copy_in, &copy_out})
+                hip_check(hipStreamCreateWithFlags(st, hipStreamNonBlocking), "hipStreamCreate");
+            for (int i = 0; i < kMaxChunks; ++i)
+                for (hipEvent_t* e : {&in_ready[i], &out_ready[i], &reconstructed[i]})
+                    hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+            for (hipEvent_t& e : marks) hip_check(hipEventCreate(&e), "hipEventCreate");
+            pool = std::make_unique<CopyPool>(copy_threads);
+        } catch (...) {
+            release();
+            throw;
+        }
+    }
+    ~Lane() { release(); }
+    Lane(const Lane&) = delete;
+    Lane& operator=(const Lane&) = delete;
+
+    static unsigned default_copy_threads() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        return hw ? std::min(16u, hw) : 4u;
+    }
+
+    void synchronize() {
+        for (hipStream_t st : {stream, copy_in, copy_out})
+            if (st) (void)hipStreamSynchronize(st);
+    }
+
+    // Pinned staging for products of up to `words` words per operand:
+    // [2][words] in (a then b), [words] out.
+    void reserve(size_t words) {
+        if (words <= pin_words) return;
+        synchronize();
+        free_pinned();
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pin_in), 2 * words * sizeof(uint32_t), hipHostMallocDefault),
+                  "pin input staging");
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pin_out), words * sizeof(uint32_t), hipHostMallocDefault),
+                  "pin output staging");
+        pin_words = words;
+    }
+
+    hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
+    hipEvent_t in_ready[kMaxChunks] = {}, out_ready[kMaxChunks] = {}, reconstructed[kMaxChunks] = {};
+    hipEvent_t marks[kEvents] = {};
+    std::unique_ptr<CopyPool> pool;
+    uint32_t* pin_in = nullptr;
+    uint32_t* pin_out = nullptr;
+    size_t pin_words = 0;
+
+private:
+    void free_pinned() {
+        if (pin_in) (void)hipHostFree(pin_in);
+        if (pin_out) (void)hipHostFree(pin_out);
+        pin_in = pin_out = nullptr;
+        pin_words = 0;
+    }
+    void release() {
+        synchronize();
+        free_pinned();
+        for (int i = 0; i < kMaxChunks; ++i)
+            for (hipEvent_t e : {in_ready[i], out_ready[i], reconstructed[i]})
+                if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : marks)
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t st : {stream, copy_in, copy_out})
+            if (st) (void)hipStreamDestroy(st);
+        stream = copy_in = copy_out = nullptr;
+    }
+};
+
+// One hipMalloc'd region handed out as first-fit blocks with coalescing on
+// release.  acquire() waits while no free block is large enough (admission by
+// memory footprint) and fails only for a request larger than the whole arena.
+class DeviceArena {
+public:
+    explicit DeviceArena(size_t bytes) : size_(bytes & ~size_t(255)) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), size_);
+        if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+        hip_check(e, "arena hipMalloc");
+        free_[0] = size_;
+    }
+    ~DeviceArena() {
+        if (base_) (void)hipFree(base_);
+    }
+    DeviceArena(const DeviceArena&) = delete;
+    DeviceArena& operator=(const DeviceArena&) = delete;
+
+    size_t capacity() const { return size_; }
+
+    char* acquire(size_t bytes) {
+        bytes = aligned_bytes(bytes);
+        if (bytes > size_) throw std::bad_alloc();
+        std::unique_lock<std::mutex> lock(mutex_);
+        for (;;) {
+            for (auto it = free_.begin(); it != free_.end(); ++it) {
+                if (it->second < bytes) continue;
+                const size_t off = it->first, size = it->second;
+                free_.erase(it);
+                if (size > bytes) free_[off + bytes] = size - bytes;
+                used_[off] = bytes;
+                return base_ + off;
+            }
+            released_.wait(lock);
+        }
+    }
+
+    void release(char* p) {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            size_t off = size_t(p - base_);
+            auto it = used_.find(off);
+            if (it == used_.end()) return;
+            size_t size = it->second;
+            used_.erase(it);
+            auto next = free_.lower_bound(off);
+            if (next != free_.end() && next->first == off + size) {
+                size += next->second;
+                next = free_.erase(next);
+            }
+            if (next != free_.begin()) {
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    prev->second += size;
+                    size = 0;
+                }
+            }
+            if (size) free_[off] = size;
+        }
+        released_.notify_all();
+    }
+
+private:
+    char* base_ = nullptr;
+    size_t size_;
+    std::mutex mutex_;
+    std::condition_variable released_;
+    std::map<size_t, size_t> free_;  // offset -> size
+    std::map<size_t, size_t> used_;  // offset -> size
+};
+
+}  // namespace pm
+
+struct pm_plan {
+    pm::HostPlan host;
+    bool device = false;
+    int device_id = 0;
+    pm::Lane* lane = nullptr;              // streams, events, pinned staging, copy workers
+    std::unique_ptr<pm::Lane> own_lane;    // set for a standalone plan
+    // Device memory: a standalone plan hipMallocs each buffer; a plan run by a
+    // server carves them, in order, out of one block of the server's arena.
+    char* block = nullptr;
+    size_t block_size = 0, block_used = 0;
+    std::vector<void*> allocations;
+    pm::DeviceTables tables;
+    pm::DeviceWork work;
+    uint32_t* io_a = nullptr;  // device copies for the host-vector entry point
+    uint32_t* io_b = nullptr;
+    uint32_t* io_c = nullptr;
+    uint64_t device_bytes = 0;
+    uint32_t* dev_beta = nullptr;  // device-built beta (Tables::kDevice), for table export
+    bool profiling = false;
+    float stage_ms[kStages] = {};
+
+    template <class T>
+    T* alloc(size_t count) {
+        const size_t bytes = aligned_bytes(count * sizeof(T));
+        void* p = nullptr;
+        if (block) {
+            if (block_used + bytes > block_size) throw std::logic_error("plan: device block smaller than its footprint");
+            p = block + block_used;
+            block_used += bytes;
+        } else {
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess) {
+                if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+                hip_check(e, "hipMalloc");
+            }
+            allocations.push_back(p);
+        }
+        device_bytes += bytes;
+        return static_cast<T*>(p);
+    }
+
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload plan");
+        return p;
+    }
+
+    ~pm_plan() {
+        if (device) {
+            (void)hipSetDevice(device_id);
+            if (lane) lane->synchronize();
+            for (void* p : allocations) (void)hipFree(p);
+        }
+    }
+
+    // Host vectors in, host vector out.  Chunk c (256-coefficient tiles
+    // [t0, t0+tc)) is copied into pinned memory by the pool, sent on copy_in,
+    // then packed and converted on the compute stream while the next chunk is
+    // still being copied; after the transforms and the CRT GEMM, results are
+    // reconstructed chunk by chunk and each chunk's D2H and host copy-out
+    // overlap the next chunk's reconstruction.
+    // b == nullptr: the b residues come from `bplanes` (a resident operand).
+    void multiply_host(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in, uint32_t* c) {
+        const pm::Shape& s = host.shape;
+        const uint32_t nops = b ? 2 : 1;
+        const uint32_t* bplanes = b ? b_planes() : bplanes_in;
+        const size_t L = s.limbs, row_bytes = L * sizeof(uint32_t);
+        if (s.trivial) {
+            std::memset(c, 0, size_t(s.n) * row_bytes);
+            return;
+        }
+        lane->reserve(size_t(s.n) * L);
+        hipStream_t stream = lane->stream, copy_in = lane->copy_in, copy_out = lane->copy_out;
+        auto& in_ready = lane->in_ready;
+        auto& out_ready = lane->out_ready;
+        auto& reconstructed = lane->reconstructed;
+        pm::CopyPool* pool = lane->pool.get();
+        uint32_t* pin_in = lane->pin_in;
+        uint32_t* pin_out = lane->pin_out;
+        const uint32_t ntiles = s.n_pad / 256;
+        const uint32_t chunks = ntiles < pm::kMaxChunks ? ntiles : pm::kMaxChunks;
+        const uint32_t per = (ntiles + chunks - 1) / chunks;
+        uint32_t* pin_a = pin_in;
+        uint32_t* pin_b = pin_in + size_t(s.n) * L;
+        auto rows_of = [&](uint32_t ch, size_t& r0, size_t& r1) {
+            r0 = std::min<size_t>(size_t(ch) * per * 256, s.n);
+            r1 = std::min<size_t>(size_t(ch + 1) * per * 256, s.n);
+        };
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            size_t r0, r1;
+            rows_of(ch, r0, r1);
+            if (r1 > r0) {
+                const size_t off = r0 * L, bytes = (r1 - r0) * row_bytes;
+                pool->copy(pin_a + off, a + off, bytes);
+                hip_check(hipMemcpyAsync(io_a + off, pin_a + off, bytes, hipMemcpyHostToDevice, copy_in), "H2D a");
+                if (b) {
+                    pool->copy(pin_b + off, b + off, bytes);
+                    hip_check(hipMemcpyAsync(io_b + off, pin_b + off, bytes, hipMemcpyHostToDevice, copy_in),
+                              "H2D b");
+                }
+            }
+            hip_check(hipEventRecord(in_ready[ch], copy_in), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(stream, in_ready[ch], 0), "hipStreamWaitEvent");
+            const uint32_t t0 = ch * per, tc = std::min(per, ntiles - t0);
+            if (tc) {
+                pm::launch_pack(s, io_a, io_b, work.packed, t0, tc, 0, nops, stream);
+                pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
+            }
+        }
+        run_middle(stream, nops, bplanes);
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            size_t r0, r1;
+            rows_of(ch, r0, r1);
+            pm::launch_reconstruct(s, tables, work.words, work.t_words, io_c, uint32_t(r0), uint32_t(r1 - r0),
+                                   stream);
+            hip_check(hipEventRecord(reconstructed[ch], stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(copy_out, reconstructed[ch], 0), "hipStreamWaitEvent");
+            if (r1 > r0)
+                hip_check(hipMemcpyAsync(pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
+                                         hipMemcpyDeviceToHost, copy_out),
+                          "D2H c");
+            hip_check(hipEventRecord(out_ready[ch], copy_out), "hipEventRecord");
+        }
+        for (uint32_t ch = 0; ch < chunks; ++ch) {
+            size_t r0, r1;
+            rows_of(ch, r0, r1);
+            hip_check(hipEventSynchronize(out_ready[ch]), "D2H c");
+            if (r1 > r0) pool->copy(c + r0 * L, pin_out + r0 * L, (r1 - r0) * row_bytes);
+        }
+        hip_check(hipStreamSynchronize(stream), "multiply");
+    }
+
+    void make_device() {
+        hip_check(hipGetDevice(&device_id), "hipGetDevice");
+        device = true;
+        if (!lane) {
+            own_lane = std::make_unique<pm::Lane>(pm::Lane::default_copy_threads());
+            lane = own_lane.get();
+        }
+        const pm::Shape& s = host.shape;
+        const size_t nl = size_t(s.n) * s.limbs;
+        io_a = alloc<uint32_t>(nl);
+        io_b = alloc<uint32_t>(nl);
+        io_c = alloc<uint32_t>(nl);
+        if (s.trivial) return;
+        pm::configure_kernels();
+        if (host.tables == pm::Tables::kDevice) {
+            build_device_tables();
+        } else {
+            tables.consts = upload(host.consts);
+            // both operands' weight matrices, each in 256x64 tiles
+            const size_t one = size_t(s.conv_rows) * s.conv_k;
+            std::vector<int8_t> tiled;
+            tiled.reserve(2 * one);
+            for (int op = 0; op < 2; ++op) {
+                std::vector<int8_t> part(host.wconv.begin() + op * one, host.wconv.begin() + (op + 1) * one);
+                std::vector<int8_t> t = pm::tile_256x64(part, s.conv_rows, s.conv_k);
+                tiled.insert(tiled.end(), t.begin(), t.end());
+            }
+            tables.wconv = upload(tiled);
+            tables.tw_fwd = reinterpret_cast<const uint2*>(upload(host.tw_fwd));
+            tables.tw_inv = reinterpret_cast<const uint2*>(upload(host.tw_inv));
+            tables.wcrt = upload(pm::tile_256x64(host.wcrt, s.crt_rows, s.crt_k));
+        }
+        tables.q = upload(host.q);
+        tables.neg_p = upload(host.neg_p_mod_q);
+        tables.corr_t = upload(host.corr_t);
+        tables.corr_e = upload(host.corr_e);
+        work.packed = alloc<int8_t>(pm::packed_operand_bytes(s));
+        // planes: [2][r_pad][n_pad].  After the row stages the b half is dead and
+        // receives the CRT operand tiles of v_k (exactly r_pad*n_pad words);
+        // after the CRT GEMM the a half is dead and holds T ([L+2][n_pad]).
+        const size_t half = size_t(s.r_pad > s.t_words ? s.r_pad : s.t_words) * s.n_pad;
+        work.planes = alloc<uint32_t>(2 * half);
+        work.rtiles = reinterpret_cast<int8_t*>(work.planes + half);
+        work.t_words = work.planes;
+        work.words = alloc<int64_t>(size_t(s.crt_words) * s.n_pad);
+        // pinned staging of the host entry points is sized here, with the rest
+        // of the plan, not inside the first product
+        lane->reserve(nl);
+    }
+
+    // Prime-set tables and CRT weights built by kernels from the O(r) and O(rL)
+    // host results (psi, reciprocals, M_k): no O(r^2) or O(r*N) host loops and no
+    // upload of the large tables.
+    void build_device_tables() {
+        const pm::Shape& s = host.shape;
+        pm::TableBuild tb;
+        tb.consts = upload(host.consts);
+        tb.psi = upload(host.psi);
+        tb.rcp = upload(host.reciprocal);
+        tb.m_mod_q = upload(host.m_mod_q);
+        tb.beta = dev_beta = alloc<uint32_t>(s.r);
+        tb.wconv = alloc<int8_t>(size_t(2) * s.conv_rows * s.conv_k);
+        tb.tw_fwd = alloc<uint32_t>(size_t(s.r) * s.n2 * 2);
+        tb.tw_inv = alloc<uint32_t>(size_t(s.r) * s.n2 * 2);
+        tb.wcrt = alloc<int8_t>(size_t(s.crt_rows) * s.crt_k);
+        pm::launch_build_tables(s, tb, lane->stream);
+        hip_check(hipStreamSynchronize(lane->stream), "build tables");
+        tables.consts = tb.consts;
+        tables.wconv = tb.wconv;
+        tables.tw_fwd = reinterpret_cast<const uint2*>(tb.tw_fwd);
+        tables.tw_inv = reinterpret_cast<const uint2*>(tb.tw_inv);
+        tables.wcrt = tb.wcrt;
+    }
+
+    // Host copies of device-built tables, in the host layouts of plan.cpp (for
+    // table export); fetched once.
+    void fetch_device_tables() {
+        const pm::Shape& s = host.shape;
+        if (host.tables != pm::Tables::kDevice || !device || s.trivial || !host.wconv.empty()) return;
+        hip_check(hipSetDevice(device_id), "hipSetDevice");
+        auto get = [&](auto& vec, const void* src, size_t count) {
+            vec.resize(count);
+            hip_check(hipMemcpy(vec.data(), src, count * sizeof(vec[0]), hipMemcpyDeviceToHost), "fetch tables");
+        };
+        get(host.consts, tables.consts, s.r);
+        get(host.beta, dev_beta, s.r);
+        get(host.tw_fwd, tables.tw_fwd, size_t(s.r) * s.n2 * 2);
+        get(host.tw_inv, tables.tw_inv, size_t(s.r) * s.n2 * 2);
+        std::vector<int8_t> tiled;
+        const size_t one = size_t(s.conv_rows) * s.conv_k;
+        get(tiled, tables.wconv, 2 * one);
+        host.wconv.resize(2 * one);
+        for (int op = 0; op < 2; ++op) {
+            std::vector<int8_t> part(tiled.begin() + op * one, tiled.begin() + (op + 1) * one);
+            std::vector<int8_t> flat = pm::untile_256x64(part, s.conv_rows, s.conv_k);
+            std::copy(flat.begin(), flat.end(), host.wconv.begin() + op * one);
+        }
+        get(tiled, tables.wcrt, size_t(s.crt_rows) * s.crt_k);
+        host.wcrt = pm::untile_256x64(tiled, s.crt_rows, s.crt_k);
+    }
+
+    // The b residues inside the work planes ([r_pad][n_pad] after a's).
+    uint32_t* b_planes() { return work.planes + size_t(host.shape.r_pad) * host.shape.n_pad; }
+
+    // Runs stages up to and including `last` (PM_STAGE_*) on device operands;
+    // b == nullptr takes the b residues from `bplanes_in` (a resident operand).
+    void run(const uint32_t* a, const uint32_t* b, uint32_t* c, int last, hipStream_t st,
+             const uint32_t* bplanes_in = nullptr) {
+        const pm::Shape& s = host.shape;
+        if (s.trivial) {
+            pm::launch_zero(c, size_t(s.n) * s.limbs, st);
+            return;
+        }
+        auto mark = [&](int i) {
+            if (profiling) hip_check(hipEventRecord(lane->marks[i], st), "hipEventRecord");
+        };
+        const uint32_t ntiles = s.n_pad / 256;
+        const uint32_t nops = b ? 2 : 1;
+        const uint32_t* bplanes = b ? b_planes() : bplanes_in;
+        mark(0);
+        pm::launch_pack(s, a, b, work.packed, 0, ntiles, 0, nops, st);
+        mark(1);
+        pm::launch_conversion(s, tables, work.packed, work.planes, 0, ntiles, 0, nops, st);
+        mark(2);
+        if (last == PM_STAGE_RESIDUES) return;
+        pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
+        mark(3);
+        pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
+        mark(4);
+        pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
+        mark(5);
+        if (last == PM_STAGE_PRODUCT) return;
+        pm::launch_crt(s, tables, work.rtiles, work.words, st);
+        mark(6);
+        if (last == PM_STAGE_WORDS) return;
+        pm::launch_reconstruct(s, tables, work.words, work.t_words, c, 0, s.n, st);
+        mark(7);
+    }
+
+    // The stages between the per-coefficient ends: transforms and the CRT GEMM.
+    void run_middle(hipStream_t st, uint32_t nops, const uint32_t* bplanes) {
+        const pm::Shape& s = host.shape;
+        pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
+        pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
+        pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
+        pm::launch_crt(s, tables, work.rtiles, work.words, st);
+    }
+
+    void collect_times() {
+        if (!profiling || host.shape.trivial) return;
+        for (int i = 0; i < kEvents - 1; ++i)
+            hip_check(hipEventElapsedTime(&stage_ms[i], lane->marks[i], lane->marks[i + 1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&stage_ms[kEvents - 1], lane->marks[0], lane->marks[kEvents - 1]),
+                  "hipEventElapsedTime");
+    }
+};
+
+// A resident factor: b's residues after conversion and the row-spanning
+// forward stages, [r][n_pad] u32 on the plan's device.  Valid only with the
+// plan that made it (b's weights carry that plan's scale beta_k).
+struct pm_operand {
+    pm_plan* plan = nullptr;
+    uint32_t* planes = nullptr;
+    ~pm_operand() {
+        if (planes) {
+            hipSetDevice(plan->device_id);
+            hipFree(planes);
+        }
+    }
+};
+
+
+// Device bytes pm_plan::make_device allocates for this host plan (the same
+// sequence of 256-byte-rounded buffers); a server reserves exactly this much.
+inline size_t plan_footprint(const pm::HostPlan& h) {
+    const pm::Shape& s = h.shape;
+    const size_t nl = size_t(s.n) * s.limbs * sizeof(uint32_t);
+    size_t t = 3 * aligned_bytes(nl);  // io_a, io_b, io_c
+    if (s.trivial) return t;
+    const size_t r = s.r;
+    t += aligned_bytes(r * sizeof(pm::PrimeConst)) + aligned_bytes(2 * r * 4) + aligned_bytes(r * 8) +
+         aligned_bytes(r * s.limbs * 4) + aligned_bytes(r * 4);
+    t += aligned_bytes(size_t(2) * s.conv_rows * s.conv_k) + 2 * aligned_bytes(r * s.n2 * 2 * 4) +
+         aligned_bytes(size_t(s.crt_rows) * s.crt_k);
+    t += 2 * aligned_bytes(size_t(s.limbs) * 4) + aligned_bytes(size_t(s.t_words) * 4) + aligned_bytes(4 * 4);
+    const size_t half = size_t(s.r_pad > s.t_words ? s.r_pad : s.t_words) * s.n_pad;
+    t += aligned_bytes(pm::packed_operand_bytes(s)) + aligned_bytes(2 * half * 4) +
+         aligned_bytes(size_t(s.crt_words) * s.n_pad * 8);
+    return t;
+}
+
+template <class F>
+pm_status guarded(F&& body) {
+    try {
+        body();
+        return PM_OK;
+    } catch (const NoDevice& e) {
+        g_last_error = e.what();
+        return PM_ERR_NO_DEVICE;
+    } catch (const DeviceError& e) {
+        g_last_error = e.what();
+        return PM_ERR_DEVICE;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return PM_ERR_INVALID_ARGUMENT;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of memory";
+        return PM_ERR_OUT_OF_MEMORY;
+    } catch (const std::runtime_error& e) {
+        g_last_error = e.what();
+        return std::string(e.what()).find("insufficient independent NTT fields") != std::string::npos
+                   ? PM_ERR_INSUFFICIENT_FIELDS
+                   : PM_ERR_INTERNAL;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return PM_ERR_INTERNAL;
+    }
+}
+
+inline void require_device(const pm_plan* plan) {
+    if (!plan) throw std::invalid_argument("null plan");
+    if (!plan->device) throw NoDevice("plan has no device state (created with pm_plan_create_host)");
+}
+

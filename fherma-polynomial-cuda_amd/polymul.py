@@ -14,6 +14,12 @@ HERE = Path(__file__).resolve().parent
 LIBRARY_PATH = HERE / "libpolymul_amd.so"
 
 PM_OK = 0
+PM_ERR_INVALID_ARGUMENT = 1
+PM_ERR_INSUFFICIENT_FIELDS = 2
+PM_ERR_DEVICE = 3
+PM_ERR_OUT_OF_MEMORY = 4
+PM_ERR_INTERNAL = 5
+PM_ERR_NO_DEVICE = 6
 STATUS_NAMES = {
     0: "PM_OK",
     1: "PM_ERR_INVALID_ARGUMENT",
@@ -64,6 +70,16 @@ class PlanInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64)]
 
 
+class Request(ctypes.Structure):
+    """pm_request: one product with its own point and host operands."""
+    _fields_ = [("n", ctypes.c_uint32), ("width", ctypes.c_uint32), ("limbs", ctypes.c_uint32),
+                ("q", ctypes.c_void_p), ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p)]
+
+
+class RequestTimes(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("service_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -97,6 +113,12 @@ def load_library(path=None):
         "pm_multiply_operand_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "pm_operand_destroy": ([vp], None),
         "pm_get_stage_times": ([vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int], ctypes.c_int),
+        "pm_server_create": ([ctypes.c_uint64, ctypes.c_uint32, plan_pp], ctypes.c_int),
+        "pm_server_submit": ([vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "pm_server_wait": ([vp, ctypes.c_uint64, ctypes.POINTER(RequestTimes)], ctypes.c_int),
+        "pm_server_destroy": ([vp], None),
+        "pm_multiply_batch": ([ctypes.POINTER(Request), ctypes.c_size_t, ctypes.c_uint64, ctypes.c_uint32,
+                               ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -252,6 +274,74 @@ class ResidentOperand:
         if getattr(self, "handle", None):
             self.plan.lib.pm_operand_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _request(n, limbs, q, a, b):
+    """Validated contiguous buffers and a pm_request for one product; returns
+    (request, c, keepalive)."""
+    q, _ = _u32(q)
+    a, _ = _u32(a)
+    b, _ = _u32(b)
+    if q.size != limbs or a.shape != (n, limbs) or b.shape != (n, limbs):
+        raise ValueError("request buffers must be q[L], a[N][L], b[N][L]")
+    c = np.empty((n, limbs), dtype=np.uint32)
+    req = Request(n, 32 * limbs, limbs, q.ctypes.data, a.ctypes.data, b.ctypes.data, c.ctypes.data)
+    return req, c, (q, a, b)
+
+
+def multiply_batch(products, device_budget_bytes, lanes=4):
+    """Many products, each (n, limbs, q, a, b) with its own q, through one
+    temporary server (pm_multiply_batch).  Returns the list of c arrays."""
+    lib = load_library()
+    built = [_request(*p) for p in products]
+    reqs = (Request * len(built))(*[b[0] for b in built])
+    statuses = (ctypes.c_int * len(built))()
+    _check(lib, lib.pm_multiply_batch(reqs, len(built), int(device_budget_bytes), int(lanes), statuses))
+    return [b[1] for b in built]
+
+
+class Server:
+    """A request server (pm_server): submit() queues a product and returns a
+    ticket at once; wait(ticket) returns (c, times) when it has finished."""
+
+    def __init__(self, device_budget_bytes, lanes=4):
+        self.lib = load_library()
+        handle = ctypes.c_void_p()
+        _check(self.lib, self.lib.pm_server_create(int(device_budget_bytes), int(lanes), ctypes.byref(handle)))
+        self.handle = handle
+        self._pending = {}
+
+    def submit(self, n, limbs, q, a, b):
+        req, c, keep = _request(n, limbs, q, a, b)
+        ticket = ctypes.c_uint64()
+        _check(self.lib, self.lib.pm_server_submit(self.handle, ctypes.byref(req), ctypes.byref(ticket)))
+        self._pending[ticket.value] = (req, c, keep)
+        return ticket.value
+
+    def wait(self, ticket):
+        times = RequestTimes()
+        status = self.lib.pm_server_wait(self.handle, ticket, ctypes.byref(times))
+        _, c, _ = self._pending.pop(ticket)
+        _check(self.lib, status)
+        return c, {"queue_ms": times.queue_ms, "service_ms": times.service_ms, "total_ms": times.total_ms}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.pm_server_destroy(self.handle)
+            self.handle = None
+            self._pending.clear()
 
     def __del__(self):
         try:

@@ -132,6 +132,48 @@ enum { PM_STAGE_RESIDUES = 1, PM_STAGE_PRODUCT = 2, PM_STAGE_WORDS = 3, PM_STAGE
 pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, int stage, void* out,
                          size_t* bytes);
 
+/* ---- Many products: the batched and served forms ----
+ * No counterpart in the reference, whose fherma_run multiplies one pair at a
+ * time (candidates/ordered-streaming/solve.cu:1226-1271).  A request carries its
+ * own point (N, W, L, q) and host operands, like one fherma_init + fherma_run.
+ * A server runs requests on `lanes` worker lanes (a compute and two copy
+ * streams, pinned staging and copy threads each, reused across requests) and
+ * one device arena of `device_budget_bytes`: a request's plan is built (tables
+ * on the device) once its exact device footprint fits in the arena, so requests
+ * are admitted by memory as earlier ones finish, and the transfers and kernels
+ * of requests on different lanes overlap.  Each result is bit-identical to
+ * pm_multiply on the same inputs.  A request's buffers must stay valid until it
+ * completes; a request larger than the whole budget fails with
+ * PM_ERR_OUT_OF_MEMORY. */
+typedef struct pm_request {
+    uint32_t n, width, limbs;
+    const uint32_t* q;  /* [L] */
+    const uint32_t* a;  /* [N][L] host */
+    const uint32_t* b;  /* [N][L] host */
+    uint32_t* c;        /* [N][L] host, written */
+} pm_request;
+
+typedef struct pm_request_times {
+    double queue_ms;    /* submitted -> admitted (host plan built, device memory reserved) */
+    double service_ms;  /* admitted -> result in c (device tables, transfers, kernels) */
+    double total_ms;    /* submitted -> result in c */
+} pm_request_times;
+
+typedef struct pm_server pm_server;
+pm_status pm_server_create(uint64_t device_budget_bytes, uint32_t lanes, pm_server** out);
+/* Queues a request (first in, first out); returns at once with its ticket. */
+pm_status pm_server_submit(pm_server* server, const pm_request* request, uint64_t* ticket);
+/* Blocks until the request has finished; returns the request's own status (its
+ * message through pm_last_error) and its times.  Each ticket is waited once. */
+pm_status pm_server_wait(pm_server* server, uint64_t ticket, pm_request_times* times);
+/* Completes every queued request, then releases the lanes and the arena. */
+void pm_server_destroy(pm_server* server);
+
+/* Every request through a temporary server; statuses (optional, [count]) gets
+ * each request's status; the return value is the first failure, or PM_OK. */
+pm_status pm_multiply_batch(const pm_request* requests, size_t count, uint64_t device_budget_bytes, uint32_t lanes,
+                            pm_status* statuses);
+
 /* Kernel timing of the most recent device multiply (pm_multiply_device or
  * pm_multiply_operand_device), when enabled: fills up to `count` entries of ms
  * (0: operand packing, 1: conversion GEMM, 2: forward column NTT, 3: row NTT +
