@@ -40,14 +40,16 @@ typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 
 // GEMM block tile: 256 weight rows x 256 coefficients x 64 bytes of K, eight
-// waves in a 2x4 grid, each owning a 128x64 slab = 8x4 MFMA 16x16 tiles
+// waves in a 4x2 grid, each owning a 64x128 slab = 4x8 MFMA 16x16 tiles
 // (128 int32 accumulators per lane; two waves per SIMD).
 constexpr uint32_t kTile = 256;
 constexpr uint32_t kTK = 64;
 constexpr uint32_t kTileBytes = kTile * kTK;  // one operand K step, 16 KB
 constexpr uint32_t kGemmThreads = 512;
 constexpr uint32_t kLoads = kTileBytes / 16 / kGemmThreads;  // 16-byte loads per thread per operand
-constexpr uint32_t kWaveM = 8, kWaveN = 4;    // 16x16 tiles per wave
+constexpr uint32_t kWaveM = 4, kWaveN = 8;    // 16x16 tiles per wave
+constexpr uint32_t kWavesN = 2;               // waves across the coefficient dimension
+constexpr uint32_t kWaveRows = kWaveM * 16, kWaveCols = kWaveN * 16;
 constexpr uint32_t kGroupM = 8;               // raster group height (tiles)
 
 void check_launch(const char* what) {
@@ -143,16 +145,18 @@ struct TiledB {  // pre-tiled signed bytes: [k steps][256 rows][64]
 
 // The K loop: A is a pre-tiled weight panel ([k steps][256][64]); LDS holds
 // two stages of (A, B) = 64 KB and one barrier ends each 64-byte K step.  Global
-// loads run two K steps ahead through two register staging sets, so a fetch has
-// two steps of MFMA work to land: while step k computes from LDS buffer k&1, step
-// k+2 is in flight and step k+1 (fetched during step k-1) goes into the other
-// buffer.  Wave (wm, wn) computes rows wm*128 .. +128 and coefficients
-// wn*64 .. +64 of the tile.
+// loads run two K steps ahead through two register staging sets.  Within step
+// k (LDS buffer k&1) a wave reads its fragments of step k first, then writes
+// step k+1 (fetched during step k-1) into the other buffer -- last read before
+// the previous barrier -- and issues the fetch of step k+2, then runs its MFMAs:
+// the LDS writes and the global loads overlap the MFMAs instead of sitting
+// between two barriers.  Wave (wm, wn) computes rows wm*64 .. +64 and
+// coefficients wn*128 .. +128 of the tile.
 template <class BLoader>
 __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel, const BLoader& bl,
                                               uint32_t ksteps, int8_t* lds, v4i (&acc)[kWaveM][kWaveN]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t wm = wave >> 2, wn = wave & 3;
+    const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     v4u ra0[kLoads], rb0[kLoads], ra1[kLoads], rb1[kLoads];
     auto load = [&](uint32_t ks, v4u (&ra)[kLoads], v4u (&rb)[kLoads]) {
         const v4u* src = reinterpret_cast<const v4u*>(a_panel + size_t(ks) * kTileBytes);
@@ -169,16 +173,18 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
         }
         bl.store(la + kTileBytes, tid, rb);
     };
-    auto compute = [&](uint32_t buf) {
+    v4i fa[kWaveM], fb[kWaveN];
+    auto fragments = [&](uint32_t buf) {
         const int8_t* la = lds + buf * 2 * kTileBytes;
         const int8_t* lb = la + kTileBytes;
-        v4i fa[kWaveM], fb[kWaveN];
 #pragma unroll
         for (uint32_t mi = 0; mi < kWaveM; ++mi)
-            fa[mi] = *reinterpret_cast<const v4i*>(la + swz(wm * 128 + mi * 16 + (lane & 15), lane >> 4));
+            fa[mi] = *reinterpret_cast<const v4i*>(la + swz(wm * kWaveRows + mi * 16 + (lane & 15), lane >> 4));
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni)
-            fb[ni] = *reinterpret_cast<const v4i*>(lb + swz(wn * 64 + ni * 16 + (lane & 15), lane >> 4));
+            fb[ni] = *reinterpret_cast<const v4i*>(lb + swz(wn * kWaveCols + ni * 16 + (lane & 15), lane >> 4));
+    };
+    auto mfmas = [&]() {
 #pragma unroll
         for (uint32_t mi = 0; mi < kWaveM; ++mi)
 #pragma unroll
@@ -192,18 +198,23 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
     __syncthreads();
     uint32_t ks = 0;
     for (; ks + 2 <= ksteps; ks += 2) {
-        // even step ks (buffer 0): set 0 is free, fetch ks+2; set 1 (step ks+1) -> buffer 1
-        if (ks + 2 < ksteps) load(ks + 2, ra0, rb0);
-        compute(0);
+        // even step ks (buffer 0): set 1 holds step ks+1 -> buffer 1; set 0 fetches ks+2
+        fragments(0);
         store(1, ra1, rb1);
+        if (ks + 2 < ksteps) load(ks + 2, ra0, rb0);
+        mfmas();
         __syncthreads();
-        // odd step ks+1 (buffer 1): set 1 is free, fetch ks+3; set 0 (step ks+2) -> buffer 0
-        if (ks + 3 < ksteps) load(ks + 3, ra1, rb1);
-        compute(1);
+        // odd step ks+1 (buffer 1): set 0 holds step ks+2 -> buffer 0; set 1 fetches ks+3
+        fragments(1);
         if (ks + 2 < ksteps) store(0, ra0, rb0);
+        if (ks + 3 < ksteps) load(ks + 3, ra1, rb1);
+        mfmas();
         __syncthreads();
     }
-    if (ks < ksteps) compute(0);  // odd step count: the last step sits in buffer 0
+    if (ks < ksteps) {  // odd step count: the last step sits in buffer 0
+        fragments(0);
+        mfmas();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -250,7 +261,7 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
                                                        uint32_t op0, uint32_t nops) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t wm = wave >> 2, wn = wave & 3;
+    const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t ntiles = n_pad / kTile, num_m = conv_rows / kTile;
     uint32_t tm, tn;
     tile_coords(num_m, nops * tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles) of nops operands
@@ -266,7 +277,7 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
     gemm_mainloop(a_panel, bl, ksteps, lds, acc);
 
     // Lane holds rows 4*(lane/16)+j (j = digit) of each 16-row tile: one prime.
-    const uint32_t m0 = tm * kTile + wm * 128, c0 = tcol * kTile + wn * 64;
+    const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tcol * kTile + wn * kWaveCols;
 #pragma unroll
     for (uint32_t mi = 0; mi < kWaveM; ++mi) {
         const uint32_t kp = (m0 + mi * 16) / 4 + (lane >> 4);
@@ -292,7 +303,7 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
                                                 uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t wm = wave >> 2, wn = wave & 3;
+    const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t ntiles = n_pad / kTile, num_m = crt_rows / kTile;
     uint32_t tm, tn;
     tile_coords(num_m, ntiles, tm, tn);
@@ -306,7 +317,7 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
         for (uint32_t ni = 0; ni < kWaveN; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
     gemm_mainloop(a_panel, bl, ksteps, lds, acc);
 
-    const uint32_t m0 = tm * kTile + wm * 128, c0 = tn * kTile + wn * 64;
+    const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tn * kTile + wn * kWaveCols;
 #pragma unroll
     for (uint32_t mi = 0; mi < kWaveM; ++mi) {
         const uint32_t word = (m0 + mi * 16) / 4 + (lane >> 4);
@@ -733,21 +744,25 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     const uint64_t kq = uint64_t(ew[2]) + (uint64_t(ew[3]) << 32) + ((uint64_t(ew[1]) + 0x80000000ull) >> 32);
 
     // Pass 1: T = sum_j 2^32j words[j] + corr_t + K*R, carried into 32-bit words.
+    // One lane owns a whole coefficient, so memory-level parallelism comes from
+    // running the loads ahead of the carry chain: two groups of kG1 words
+    // alternate, one in flight while the other is carried.
     const uint32_t sw = args.s_shift / 32, sb = args.s_shift % 32;
     uint32_t top[3] = {0, 0, 0};
     {
-        // Loads are issued 8 words ahead of the carry chain (one lane owns a
-        // whole coefficient, so memory-level parallelism has to come from here).
+        constexpr uint32_t kG1 = 16;
+        int64_t da[kG1], db[kG1];
         int64_t carry = 0;
-        for (uint32_t j0 = 0; j0 < tw; j0 += 8) {
-            int64_t d[8];
+        auto fetch = [&](int64_t (&d)[kG1], uint32_t j0) {
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
+            for (uint32_t u = 0; u < kG1; ++u) {
                 const uint32_t j = j0 + u;
                 d[u] = j <= L && j < tw ? words[size_t(j) * ps + ic] : 0;
             }
+        };
+        auto carry_group = [&](const int64_t (&d)[kG1], uint32_t j0) {
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
+            for (uint32_t u = 0; u < kG1; ++u) {
                 const uint32_t j = j0 + u;
                 if (j >= tw) break;
                 int64_t x = d[u] + int64_t(corr_t[j]) + carry;
@@ -755,8 +770,19 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
                 const uint32_t tj = uint32_t(uint64_t(x));
                 carry = x >> 32;
                 if (valid) tsc[size_t(j) * ps + ic] = tj;
-                if (j >= sw && j < sw + 3) top[j - sw] = tj;
+                // the three words Barrett reads; compares, not a runtime index (which would spill)
+                if (j == sw) top[0] = tj;
+                if (j == sw + 1) top[1] = tj;
+                if (j == sw + 2) top[2] = tj;
             }
+        };
+        fetch(da, 0);
+        for (uint32_t j0 = 0; j0 < tw; j0 += 2 * kG1) {
+            if (j0 + kG1 < tw) fetch(db, j0 + kG1);
+            carry_group(da, j0);
+            if (j0 + kG1 >= tw) break;
+            if (j0 + 2 * kG1 < tw) fetch(da, j0 + 2 * kG1);
+            carry_group(db, j0 + kG1);
         }
     }
     const unsigned __int128 topv = (unsigned __int128)top[0] | ((unsigned __int128)top[1] << 32) |
@@ -764,39 +790,54 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     const uint64_t ts = uint64_t(topv >> sb);
     const uint64_t qe = uint64_t(((unsigned __int128)ts * args.mu) >> args.k_shift);
 
-    // Pass 2: rem = T - qe*q (in [0, 2q)), emitted through an LDS transpose;
-    // a second borrow chain tracks rem - q.
+    // Pass 2: rem = T - qe*q (in [0, 2q)), emitted through an LDS transpose of 64
+    // words at a time; a second borrow chain tracks rem - q.  T is read in
+    // halves of 32 words, the next half in flight while the current is reduced.
     uint64_t pc = 0;
     uint32_t bo = 0, bo2 = 0;
-    for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
-        const uint32_t jend = j0 + 64 < tw ? j0 + 64 : tw;
-        for (uint32_t g = j0; g < jend; g += 8) {
-            uint32_t tv[8];
+    {
+        constexpr uint32_t kH = 32;
+        uint32_t ta[kH], tb[kH];
+        auto fetch = [&](uint32_t (&t)[kH], uint32_t j0) {
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) tv[u] = g + u < jend ? tsc[size_t(g + u) * ps + ic] : 0;
+            for (uint32_t u = 0; u < kH; ++u) t[u] = j0 + u < tw ? tsc[size_t(j0 + u) * ps + ic] : 0;
+        };
+        auto reduce_half = [&](const uint32_t (&t)[kH], uint32_t j0, uint32_t col0) {
 #pragma unroll
-            for (uint32_t u = 0; u < 8; ++u) {
-                const uint32_t j = g + u;
-                if (j >= jend) break;
+            for (uint32_t u = 0; u < kH; ++u) {
+                const uint32_t j = j0 + u;
+                if (j >= tw) break;
                 const uint32_t qj = j < L ? q[j] : 0;
                 const unsigned __int128 pr = (unsigned __int128)qe * qj + pc;
                 const uint32_t x = uint32_t(pr);
                 pc = uint64_t(pr >> 32);
-                const uint64_t d = uint64_t(tv[u]) - x - bo;
+                const uint64_t d = uint64_t(t[u]) - x - bo;
                 const uint32_t rj = uint32_t(d);
                 bo = uint32_t(d >> 63);
                 const uint64_t d2 = uint64_t(rj) - qj - bo2;
                 bo2 = uint32_t(d2 >> 63);
-                tile[lane][j - j0] = rj;
+                tile[lane][col0 + u] = rj;
             }
+        };
+        auto flush = [&](uint32_t j0) {  // row-wise coalesced store of c[i0+row][j0 .. j0+64) for j < L
+            __syncthreads();
+            const uint32_t jend = j0 + 64 < tw ? j0 + 64 : tw;
+            for (uint32_t row = 0; row < 64; ++row) {
+                const uint32_t gi = i0 + row, j = j0 + lane;
+                if (gi < args.i_end && j < jend && j < L) c[size_t(gi) * L + j] = tile[row][lane];
+            }
+            __syncthreads();
+        };
+        fetch(ta, 0);
+        for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
+            if (j0 + kH < tw) fetch(tb, j0 + kH);
+            reduce_half(ta, j0, 0);
+            if (j0 + kH < tw) {
+                if (j0 + 64 < tw) fetch(ta, j0 + 64);
+                reduce_half(tb, j0 + kH, kH);
+            }
+            flush(j0);
         }
-        __syncthreads();
-        // row-wise coalesced store: c[i0+row][j0 .. jend) for j < L
-        for (uint32_t row = 0; row < 64; ++row) {
-            const uint32_t gi = i0 + row, j = j0 + lane;
-            if (gi < args.i_end && j < jend && j < L) c[size_t(gi) * L + j] = tile[row][lane];
-        }
-        __syncthreads();
     }
     // rem >= q (the estimate was one short): subtract q once more, in place.
     __syncthreads();
