@@ -58,6 +58,23 @@ pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint3
     });
 }
 
+pm_status pm_multiply_begin(pm_plan* plan, const uint32_t* a, const uint32_t* b) {
+    return guarded([&] {
+        require_device(plan);
+        if (!a || !b) throw std::invalid_argument("null operand");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        plan->multiply_begin(a, b, nullptr);
+    });
+}
+
+pm_status pm_multiply_finish(pm_plan* plan, uint32_t* c) {
+    return guarded([&] {
+        require_device(plan);
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        plan->multiply_finish(c);
+    });
+}
+
 pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream) {
     return guarded([&] {
         require_device(plan);

@@ -3,6 +3,11 @@
 // bandwidth.  Plays the role of the reference's copy lanes
 // (candidates/ordered-streaming/streaming_team.h:11-60, cpu_copy.h:14-39),
 // without CPU pinning or non-temporal stores.
+//
+// A product hands the pool one job per chunk (both operands' slices at once).
+// While a product is in flight (set_active(true)) idle workers spin on the job
+// counter, so a hand-off costs about a microsecond instead of a condition-
+// variable wake-up; otherwise they sleep.
 #pragma once
 
 #include <algorithm>
@@ -18,6 +23,12 @@ namespace pm {
 
 class CopyPool {
 public:
+    struct Segment {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+
     explicit CopyPool(unsigned threads) : parts_(threads < 1 ? 1 : threads) {
         for (unsigned t = 1; t < parts_; ++t) workers_.emplace_back([this, t] { loop(t); });
     }
@@ -25,8 +36,8 @@ public:
     ~CopyPool() {
         {
             std::lock_guard<std::mutex> lock(mutex_);
-            stop_ = true;
-            ++generation_;
+            stop_.store(true, std::memory_order_release);
+            generation_.fetch_add(1, std::memory_order_acq_rel);
         }
         wake_.notify_all();
         for (auto& w : workers_) w.join();
@@ -35,63 +46,93 @@ public:
     CopyPool(const CopyPool&) = delete;
     CopyPool& operator=(const CopyPool&) = delete;
 
-    // Copies bytes from src to dst; the calling thread takes one share.
-    // Returns when every share is done.  Not reentrant: one job at a time.
-    void copy(void* dst, const void* src, size_t bytes) {
-        if (bytes < (size_t(1) << 20) || parts_ == 1) {
-            std::memcpy(dst, src, bytes);
-            return;
-        }
-        {
-            std::lock_guard<std::mutex> lock(mutex_);
-            dst_ = static_cast<char*>(dst);
-            src_ = static_cast<const char*>(src);
-            bytes_ = bytes;
-            pending_.store(parts_ - 1, std::memory_order_relaxed);
-            ++generation_;
-        }
-        wake_.notify_all();
-        share(0);
-        std::unique_lock<std::mutex> lock(mutex_);
-        done_.wait(lock, [this] { return pending_.load(std::memory_order_acquire) == 0; });
-    }
-
     unsigned threads() const { return parts_; }
 
+    // Spin (true) or sleep (false) while idle.  Not reentrant with copy().
+    void set_active(bool on) {
+        {
+            std::lock_guard<std::mutex> lock(mutex_);
+            active_.store(on, std::memory_order_release);
+        }
+        if (on) wake_.notify_all();
+    }
+
+    void copy(void* dst, const void* src, size_t bytes) {
+        const Segment s{dst, src, bytes};
+        copy(&s, 1);
+    }
+
+    // Copies every segment; each is split into 4 KB-aligned shares, one per
+    // thread, and the calling thread takes share 0.  Returns when all are done.
+    // Not reentrant: one job at a time.
+    void copy(const Segment* segs, size_t count) {
+        size_t total = 0;
+        for (size_t i = 0; i < count; ++i) total += segs[i].bytes;
+        if (total < (size_t(1) << 20) || parts_ == 1) {
+            for (size_t i = 0; i < count; ++i) std::memcpy(segs[i].dst, segs[i].src, segs[i].bytes);
+            return;
+        }
+        segs_.assign(segs, segs + count);
+        pending_.store(parts_ - 1, std::memory_order_relaxed);
+        if (active_.load(std::memory_order_acquire)) {
+            generation_.fetch_add(1, std::memory_order_release);  // spinning workers see it
+        } else {
+            {
+                std::lock_guard<std::mutex> lock(mutex_);
+                generation_.fetch_add(1, std::memory_order_release);
+            }
+            wake_.notify_all();
+        }
+        share(0);
+        while (pending_.load(std::memory_order_acquire) != 0) pause();
+    }
+
 private:
+    static void pause() { __builtin_ia32_pause(); }
+
     void share(unsigned t) {
-        // 4 KB-aligned slices so neighbouring shares never split a page
-        const size_t pages = (bytes_ + 4095) / 4096;
-        const size_t first = pages * t / parts_ * 4096, last = std::min(bytes_, pages * (t + 1) / parts_ * 4096);
-        if (last > first) std::memcpy(dst_ + first, src_ + first, last - first);
+        for (const Segment& s : segs_) {
+            const size_t pages = (s.bytes + 4095) / 4096;
+            const size_t first = pages * t / parts_ * 4096;
+            const size_t last = std::min(s.bytes, pages * (t + 1) / parts_ * 4096);
+            if (last > first)
+                std::memcpy(static_cast<char*>(s.dst) + first, static_cast<const char*>(s.src) + first, last - first);
+        }
     }
 
     void loop(unsigned t) {
+        // the counter's initial value, not a fresh load: a job published before
+        // this thread got here must still be taken
         size_t seen = 0;
         for (;;) {
-            {
+            // spin while the pool is active, sleep otherwise
+            while (generation_.load(std::memory_order_acquire) == seen && active_.load(std::memory_order_acquire) &&
+                   !stop_.load(std::memory_order_acquire))
+                pause();
+            if (generation_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_acquire)) {
                 std::unique_lock<std::mutex> lock(mutex_);
-                wake_.wait(lock, [&] { return generation_ != seen; });
-                seen = generation_;
-                if (stop_) return;
+                wake_.wait(lock, [&] {
+                    return generation_.load(std::memory_order_acquire) != seen ||
+                           active_.load(std::memory_order_acquire) || stop_.load(std::memory_order_acquire);
+                });
+                if (generation_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_acquire))
+                    continue;  // woken to spin
             }
+            if (stop_.load(std::memory_order_acquire)) return;
+            seen = generation_.load(std::memory_order_acquire);
             share(t);
-            if (pending_.fetch_sub(1, std::memory_order_acq_rel) == 1) {
-                std::lock_guard<std::mutex> lock(mutex_);
-                done_.notify_one();
-            }
+            pending_.fetch_sub(1, std::memory_order_acq_rel);
         }
     }
 
     const unsigned parts_;
     std::vector<std::thread> workers_;
     std::mutex mutex_;
-    std::condition_variable wake_, done_;
-    size_t generation_ = 0;
-    bool stop_ = false;
-    char* dst_ = nullptr;
-    const char* src_ = nullptr;
-    size_t bytes_ = 0;
+    std::condition_variable wake_;
+    std::atomic<size_t> generation_{0};
+    std::atomic<bool> active_{false};
+    std::atomic<bool> stop_{false};
+    std::vector<Segment> segs_;
     std::atomic<unsigned> pending_{0};
 };
 

@@ -6,6 +6,7 @@
 #include "fherma.h"
 #include "polymul.h"
 
+#include <future>
 #include <string>
 
 namespace {
@@ -37,8 +38,20 @@ fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
         throw std::runtime_error("fherma_run: inputs must hold N*L = " + std::to_string(count) + " limbs");
     fherma::Outputs out;
     out.c.shape = {int64_t(info.n), int64_t(info.limbs)};
-    out.c.data.resize(count);
-    status = pm_multiply(plan, in.a.data.data(), in.b.data.data(), out.c.data.data());
+    // The returned vector is zero-filled by its constructor (N*L words, first
+    // touch of fresh pages); that is built on a helper thread while the inputs
+    // go in and the device works, as the reference does with its output worker
+    // (candidates/ordered-streaming/output_worker.h:17-75).
+    auto output = std::async(std::launch::async, [&out, count] { out.c.data.resize(count); });
+    status = pm_multiply_begin(plan, in.a.data.data(), in.b.data.data());
+    try {
+        output.get();  // joins, and rethrows an allocation failure
+    } catch (...) {
+        if (status == PM_OK) pm_multiply_finish(plan, nullptr);  // leave the plan ready for the next call
+        throw;
+    }
+    if (status != PM_OK) fail("fherma_run", status);
+    status = pm_multiply_finish(plan, out.c.data.data());
     if (status != PM_OK) fail("fherma_run", status);
     return out;
 }

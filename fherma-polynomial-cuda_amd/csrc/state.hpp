@@ -75,9 +75,11 @@ public:
     Lane(const Lane&) = delete;
     Lane& operator=(const Lane&) = delete;
 
+    // Copies into pinned staging peak at about 8 threads (more threads only
+    // contend for the same memory bandwidth), and half the host stays free.
     static unsigned default_copy_threads() {
         const unsigned hw = std::thread::hardware_concurrency();
-        return hw ? std::min(16u, hw) : 4u;
+        return hw ? std::max(1u, std::min(8u, hw / 2)) : 4u;
     }
 
     void synchronize() {
@@ -217,6 +219,7 @@ struct pm_plan {
     uint32_t* io_c = nullptr;
     uint64_t device_bytes = 0;
     uint32_t* dev_beta = nullptr;  // device-built beta (Tables::kDevice), for table export
+    bool pending = false;          // multiply_begin done, multiply_finish not yet
     bool profiling = false;
     float stage_ms[kStages] = {};
 
@@ -255,81 +258,119 @@ struct pm_plan {
         }
     }
 
-    // Host vectors in, host vector out.  Chunk c (256-coefficient tiles
-    // [t0, t0+tc)) is copied into pinned memory by the pool, sent on copy_in,
-    // then packed and converted on the compute stream while the next chunk is
-    // still being copied; after the transforms and the CRT GEMM, results are
-    // reconstructed chunk by chunk and each chunk's D2H and host copy-out
-    // overlap the next chunk's reconstruction.
-    // b == nullptr: the b residues come from `bplanes` (a resident operand).
-    void multiply_host(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in, uint32_t* c) {
+    // Coefficient chunks of the host pipeline, in whole 256-coefficient tiles:
+    // chunk ch covers rows [r0, r1).
+    struct Chunking {
+        uint32_t count, per;  // chunks, tiles per chunk
+        void rows(uint32_t ch, size_t n, size_t& r0, size_t& r1) const {
+            r0 = std::min<size_t>(size_t(ch) * per * 256, n);
+            r1 = std::min<size_t>(size_t(ch + 1) * per * 256, n);
+        }
+    };
+    Chunking chunking(uint32_t limit) const {
+        const uint32_t ntiles = host.shape.n_pad / 256;
+        const uint32_t count = std::max(1u, std::min(ntiles, limit));
+        return Chunking{count, (ntiles + count - 1) / count};
+    }
+    // First half of a host-vector product.  Per coefficient chunk, the copy
+    // workers move a and b (one job) from the caller's pageable vectors into
+    // pinned staging, the H2D goes on a copy stream, and the compute stream
+    // waits on an event before packing and converting that chunk: the host
+    // copy of chunk k+1 overlaps the H2D of chunk k and the conversion of chunk
+    // k-1.  After the transforms and the CRT GEMM, each chunk is reconstructed
+    // and sent D2H on the second copy stream while the next is reconstructed.
+    // Returns with the device busy; multiply_finish collects c.
+    // b == nullptr: the b residues come from `bplanes_in` (a resident operand).
+    void multiply_begin(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in) {
         const pm::Shape& s = host.shape;
         const uint32_t nops = b ? 2 : 1;
         const uint32_t* bplanes = b ? b_planes() : bplanes_in;
         const size_t L = s.limbs, row_bytes = L * sizeof(uint32_t);
+        if (pending) throw std::invalid_argument("the plan's previous product has not been finished");
         if (s.trivial) {
-            std::memset(c, 0, size_t(s.n) * row_bytes);
+            pending = true;
             return;
         }
         lane->reserve(size_t(s.n) * L);
         hipStream_t stream = lane->stream, copy_in = lane->copy_in, copy_out = lane->copy_out;
-        auto& in_ready = lane->in_ready;
-        auto& out_ready = lane->out_ready;
-        auto& reconstructed = lane->reconstructed;
         pm::CopyPool* pool = lane->pool.get();
-        uint32_t* pin_in = lane->pin_in;
-        uint32_t* pin_out = lane->pin_out;
-        const uint32_t ntiles = s.n_pad / 256;
-        const uint32_t chunks = ntiles < pm::kMaxChunks ? ntiles : pm::kMaxChunks;
-        const uint32_t per = (ntiles + chunks - 1) / chunks;
-        uint32_t* pin_a = pin_in;
-        uint32_t* pin_b = pin_in + size_t(s.n) * L;
-        auto rows_of = [&](uint32_t ch, size_t& r0, size_t& r1) {
-            r0 = std::min<size_t>(size_t(ch) * per * 256, s.n);
-            r1 = std::min<size_t>(size_t(ch + 1) * per * 256, s.n);
-        };
-        for (uint32_t ch = 0; ch < chunks; ++ch) {
+        uint32_t* pin_a = lane->pin_in;
+        uint32_t* pin_b = lane->pin_in + size_t(s.n) * L;
+        pool->set_active(true);  // workers spin between the chunk jobs of this product
+        struct Idle {
+            pm::CopyPool* pool;
+            bool armed = true;
+            ~Idle() {
+                if (armed) pool->set_active(false);
+            }
+        } idle{pool};
+        const Chunking chunks = chunking(pm::kMaxChunks);
+        for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
-            rows_of(ch, r0, r1);
+            chunks.rows(ch, s.n, r0, r1);
             if (r1 > r0) {
                 const size_t off = r0 * L, bytes = (r1 - r0) * row_bytes;
-                pool->copy(pin_a + off, a + off, bytes);
+                const pm::CopyPool::Segment segs[2] = {{pin_a + off, a + off, bytes}, {pin_b + off, b + off, bytes}};
+                pool->copy(segs, b ? 2 : 1);
                 hip_check(hipMemcpyAsync(io_a + off, pin_a + off, bytes, hipMemcpyHostToDevice, copy_in), "H2D a");
-                if (b) {
-                    pool->copy(pin_b + off, b + off, bytes);
+                if (b)
                     hip_check(hipMemcpyAsync(io_b + off, pin_b + off, bytes, hipMemcpyHostToDevice, copy_in),
                               "H2D b");
-                }
             }
-            hip_check(hipEventRecord(in_ready[ch], copy_in), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(stream, in_ready[ch], 0), "hipStreamWaitEvent");
-            const uint32_t t0 = ch * per, tc = std::min(per, ntiles - t0);
+            hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
+            const uint32_t t0 = ch * chunks.per, tc = std::min(chunks.per, s.n_pad / 256 - t0);
             if (tc) {
                 pm::launch_pack(s, io_a, io_b, work.packed, t0, tc, 0, nops, stream);
                 pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
             }
         }
         run_middle(stream, nops, bplanes);
-        for (uint32_t ch = 0; ch < chunks; ++ch) {
+        for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
-            rows_of(ch, r0, r1);
+            chunks.rows(ch, s.n, r0, r1);
             pm::launch_reconstruct(s, tables, work.words, work.t_words, io_c, uint32_t(r0), uint32_t(r1 - r0),
                                    stream);
-            hip_check(hipEventRecord(reconstructed[ch], stream), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(copy_out, reconstructed[ch], 0), "hipStreamWaitEvent");
+            hip_check(hipEventRecord(lane->reconstructed[ch], stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(copy_out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
             if (r1 > r0)
-                hip_check(hipMemcpyAsync(pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
+                hip_check(hipMemcpyAsync(lane->pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
                                          hipMemcpyDeviceToHost, copy_out),
                           "D2H c");
-            hip_check(hipEventRecord(out_ready[ch], copy_out), "hipEventRecord");
+            hip_check(hipEventRecord(lane->out_ready[ch], copy_out), "hipEventRecord");
         }
-        for (uint32_t ch = 0; ch < chunks; ++ch) {
+        pending = true;  // only once every step is queued: a failure above leaves the plan usable
+        idle.armed = false;  // the pool keeps spinning until multiply_finish
+    }
+
+    // Second half: waits for each chunk's D2H and copies it into c (c == nullptr:
+    // waits and discards the result).
+    void multiply_finish(uint32_t* c) {
+        if (!pending) throw std::invalid_argument("no product has been begun on this plan");
+        pending = false;
+        const pm::Shape& s = host.shape;
+        const size_t L = s.limbs, row_bytes = L * sizeof(uint32_t);
+        if (s.trivial) {
+            if (c) std::memset(c, 0, size_t(s.n) * row_bytes);
+            return;
+        }
+        struct Idle {
+            pm::CopyPool* pool;
+            ~Idle() { pool->set_active(false); }
+        } idle{lane->pool.get()};
+        const Chunking chunks = chunking(pm::kMaxChunks);
+        for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
-            rows_of(ch, r0, r1);
-            hip_check(hipEventSynchronize(out_ready[ch]), "D2H c");
-            if (r1 > r0) pool->copy(c + r0 * L, pin_out + r0 * L, (r1 - r0) * row_bytes);
+            chunks.rows(ch, s.n, r0, r1);
+            hip_check(hipEventSynchronize(lane->out_ready[ch]), "D2H c");
+            if (c && r1 > r0) lane->pool->copy(c + r0 * L, lane->pin_out + r0 * L, (r1 - r0) * row_bytes);
         }
-        hip_check(hipStreamSynchronize(stream), "multiply");
+        hip_check(hipStreamSynchronize(lane->stream), "multiply");
+    }
+
+    void multiply_host(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in, uint32_t* c) {
+        multiply_begin(a, b, bplanes_in);
+        multiply_finish(c);
     }
 
     void make_device() {

@@ -102,6 +102,8 @@ def load_library(path=None):
         "pm_plan_destroy": ([vp], None),
         "pm_multiply": ([vp, vp, vp, vp], ctypes.c_int),
         "pm_multiply_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "pm_multiply_begin": ([vp, vp, vp], ctypes.c_int),
+        "pm_multiply_finish": ([vp, vp], ctypes.c_int),
         "pm_last_error": ([], ctypes.c_char_p),
         "pm_plan_get_info": ([vp, ctypes.POINTER(PlanInfo)], ctypes.c_int),
         "pm_plan_get_primes": ([vp, u32p, ctypes.c_size_t], ctypes.c_int),
@@ -209,6 +211,24 @@ class Plan:
         c = np.empty(expect, dtype=np.uint32)
         _check(self.lib, self.lib.pm_multiply(self.handle, a.ctypes.data_as(ctypes.c_void_p),
                                               b.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p)))
+        return c
+
+    def multiply_begin(self, a, b):
+        """First half of multiply(): inputs in, device work queued."""
+        a, _ = _u32(a)
+        b, _ = _u32(b)
+        expect = (self.n, self.limbs)
+        if a.shape != expect or b.shape != expect:
+            raise ValueError("operands must be [N][L] = %s" % (expect,))
+        _check(self.lib, self.lib.pm_multiply_begin(self.handle, a.ctypes.data_as(ctypes.c_void_p),
+                                                    b.ctypes.data_as(ctypes.c_void_p)))
+
+    def multiply_finish(self, out=None):
+        """Second half: waits and returns c (written into `out` if given)."""
+        c = out if out is not None else np.empty((self.n, self.limbs), dtype=np.uint32)
+        if c.shape != (self.n, self.limbs) or c.dtype != np.uint32 or not c.flags.c_contiguous:
+            raise ValueError("out must be a contiguous uint32 [N][L] array")
+        _check(self.lib, self.lib.pm_multiply_finish(self.handle, c.ctypes.data_as(ctypes.c_void_p)))
         return c
 
     def multiply_device(self, a_ptr, b_ptr, c_ptr, stream=None):

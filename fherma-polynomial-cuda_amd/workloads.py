@@ -101,3 +101,20 @@ def operands_for(n, q_words, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     q_words = np.ascontiguousarray(q_words, dtype=np.uint32)
     return uniform_below(q_words, n, rng), uniform_below(q_words, n, rng)
+
+
+def write_point_dir(root, n, width, limbs, q_words, cases):
+    """A case directory in the harness's layout (src/main.cpp): manifest.json,
+    point/q.bin, cases/NNNNNN/{a,b}.bin; `cases` is a list of (a, b)."""
+    import json
+    from pathlib import Path
+
+    root = Path(root)
+    (root / "point").mkdir(parents=True, exist_ok=True)
+    (root / "manifest.json").write_text(json.dumps({"point": {"N": n, "W": width, "L": limbs}, "cases": len(cases)}))
+    np.ascontiguousarray(q_words, dtype="<u4").tofile(root / "point" / "q.bin")
+    for i, (a, b) in enumerate(cases):
+        d = root / "cases" / ("%06d" % i)
+        d.mkdir(parents=True, exist_ok=True)
+        np.ascontiguousarray(a, dtype="<u4").tofile(d / "a.bin")
+        np.ascontiguousarray(b, dtype="<u4").tofile(d / "b.bin")

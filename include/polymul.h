@@ -70,6 +70,16 @@ void pm_plan_destroy(pm_plan* plan);
  * device->host transfers, like the reference's timed window. */
 pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c);
 
+/* pm_multiply in two halves, for callers that prepare the output buffer while
+ * the device works (the harness's fherma_run returns a newly allocated, zeroed
+ * vector; the reference builds it concurrently too, output_worker.h:17-75).
+ * pm_multiply_begin copies a and b in and queues every device step; it returns
+ * with the device busy.  pm_multiply_finish waits and writes c (c == NULL: waits
+ * and discards the result).  a and b may be released after begin returns; no
+ * other product may use the plan in between. */
+pm_status pm_multiply_begin(pm_plan* plan, const uint32_t* a, const uint32_t* b);
+pm_status pm_multiply_finish(pm_plan* plan, uint32_t* c);
+
 /* Device-resident product: a, b, c are device pointers to [N][L] words on the
  * plan's device; stream is a hipStream_t (NULL = the plan's own stream).
  * No counterpart in the reference (its operands always arrive in host

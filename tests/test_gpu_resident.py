@@ -66,3 +66,20 @@ def test_device_entry_matches_host_entry(polymul, workloads, oracle, device_memo
                 device_memory.free(ptr)
         if name == "small":
             assert np.array_equal(expect, oracle.kronecker_gmp(a, b, q))
+
+
+def test_split_entry_matches_multiply(polymul, workloads):
+    """pm_multiply_begin / pm_multiply_finish give pm_multiply's c; a second
+    begin before finish is refused; finish(NULL) discards and frees the plan."""
+    print("[gpu-case] begin/finish", flush=True)
+    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        expect = plan.multiply(a, b)
+        plan.multiply_begin(a, b)
+        with pytest.raises(polymul.PolymulError) as err:
+            plan.multiply_begin(a, b)
+        assert err.value.status == polymul.PM_ERR_INVALID_ARGUMENT
+        assert np.array_equal(plan.multiply_finish(), expect)
+        plan.multiply_begin(b, a)
+        polymul.load_library().pm_multiply_finish(plan.handle, None)  # discard
+        assert np.array_equal(plan.multiply(a, b), expect)
