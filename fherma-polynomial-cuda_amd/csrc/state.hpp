@@ -267,6 +267,14 @@ struct pm_plan {
             r1 = std::min<size_t>(size_t(ch + 1) * per * 256, n);
         }
     };
+    // About one chunk per 6 MB of operand, 1 .. kMaxChunks: each chunk costs some
+    // tens of microseconds of copy-engine and event hand-offs, which small
+    // products feel, while large ones want the overlap of many chunks.
+    uint32_t chunk_limit() const {
+        const size_t bytes = size_t(host.shape.n) * host.shape.limbs * sizeof(uint32_t);
+        const size_t per = size_t(6) << 20;
+        return uint32_t(std::max<size_t>(1, std::min<size_t>(pm::kMaxChunks, (bytes + per - 1) / per)));
+    }
     Chunking chunking(uint32_t limit) const {
         const uint32_t ntiles = host.shape.n_pad / 256;
         const uint32_t count = std::max(1u, std::min(ntiles, limit));
@@ -304,7 +312,7 @@ struct pm_plan {
                 if (armed) pool->set_active(false);
             }
         } idle{pool};
-        const Chunking chunks = chunking(pm::kMaxChunks);
+        const Chunking chunks = chunking(chunk_limit());
         for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
             chunks.rows(ch, s.n, r0, r1);
@@ -358,7 +366,7 @@ struct pm_plan {
             pm::CopyPool* pool;
             ~Idle() { pool->set_active(false); }
         } idle{lane->pool.get()};
-        const Chunking chunks = chunking(pm::kMaxChunks);
+        const Chunking chunks = chunking(chunk_limit());
         for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
             chunks.rows(ch, s.n, r0, r1);
