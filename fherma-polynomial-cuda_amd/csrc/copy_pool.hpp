@@ -2,7 +2,13 @@
 // buffers <-> pinned staging), so a chunk copy uses several cores' memory
 // bandwidth.  Plays the role of the reference's copy lanes
 // (candidates/ordered-streaming/streaming_team.h:11-60, cpu_copy.h:14-39),
-// without CPU pinning or non-temporal stores.
+// without CPU pinning.
+//
+// Stores are non-temporal (AVX2 streaming stores where the CPU has them): the
+// destinations -- pinned staging that a DMA engine reads next, the caller's
+// output vector -- are not read back by this thread, so a cached store's
+// read-for-ownership would only add a third of the bytes to host DRAM traffic,
+// which is what bounds these copies.
 //
 // A product hands the pool one job per chunk (both operands' slices at once).
 // While a product is in flight (set_active(true)) idle workers spin on the job
@@ -14,12 +20,43 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
+#include <immintrin.h>
 #include <mutex>
 #include <thread>
 #include <vector>
 
 namespace pm {
+
+__attribute__((target("avx2"))) inline void copy_streaming_avx2(char* out, const char* in, size_t bytes) {
+    const size_t head = std::min(bytes, size_t((32 - (reinterpret_cast<uintptr_t>(out) & 31)) & 31));
+    std::memcpy(out, in, head);
+    out += head, in += head, bytes -= head;
+    size_t i = 0;
+    for (; i + 128 <= bytes; i += 128) {
+        const __m256i v0 = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(in + i));
+        const __m256i v1 = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(in + i + 32));
+        const __m256i v2 = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(in + i + 64));
+        const __m256i v3 = _mm256_loadu_si256(reinterpret_cast<const __m256i*>(in + i + 96));
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(out + i), v0);
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(out + i + 32), v1);
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(out + i + 64), v2);
+        _mm256_stream_si256(reinterpret_cast<__m256i*>(out + i + 96), v3);
+    }
+    std::memcpy(out + i, in + i, bytes - i);
+}
+
+// Copy with non-temporal stores, ordered (sfence) before the caller publishes
+// completion to another thread or to a DMA engine.
+inline void copy_streaming(void* dst, const void* src, size_t bytes) {
+    static const bool avx2 = __builtin_cpu_supports("avx2");
+    if (avx2)
+        copy_streaming_avx2(static_cast<char*>(dst), static_cast<const char*>(src), bytes);
+    else
+        std::memcpy(dst, src, bytes);
+    _mm_sfence();
+}
 
 class CopyPool {
 public:
@@ -96,7 +133,8 @@ private:
             const size_t first = pages * t / parts_ * 4096;
             const size_t last = std::min(s.bytes, pages * (t + 1) / parts_ * 4096);
             if (last > first)
-                std::memcpy(static_cast<char*>(s.dst) + first, static_cast<const char*>(s.src) + first, last - first);
+                copy_streaming(static_cast<char*>(s.dst) + first, static_cast<const char*>(s.src) + first,
+                               last - first);
         }
     }
 

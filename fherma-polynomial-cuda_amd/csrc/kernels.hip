@@ -829,6 +829,38 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
             __syncthreads();
         };
         fetch(ta, 0);
+        if (tw <= 64) {
+            // One group holds the whole remainder: finish it in LDS (subtract q
+            // when rem >= q) and store the block's rows as one contiguous run of
+            // 256-byte-aligned words.  c is written once and never read back, so
+            // it may be pinned host memory.
+            if (kH < tw) fetch(tb, kH);
+            reduce_half(ta, 0, 0);
+            if (kH < tw) reduce_half(tb, kH, kH);
+            if (bo2 == 0) {
+                uint32_t borrow = 0;
+                for (uint32_t j = 0; j < L; ++j) {
+                    const uint64_t d = uint64_t(tile[lane][j]) - q[j] - borrow;
+                    tile[lane][j] = uint32_t(d);
+                    borrow = uint32_t(d >> 63);
+                }
+            }
+            __syncthreads();
+            const uint32_t rows = args.i_end - i0 < 64 ? args.i_end - i0 : 64;
+            uint32_t* out = c + size_t(i0) * L;
+            const uint32_t dq = 64 / L, dr = 64 % L;  // (row, col) of word k, stepped by 64 words
+            uint32_t row = lane / L, col = lane % L;
+            for (uint32_t k = lane; k < rows * L; k += 64) {
+                out[k] = tile[row][col];
+                row += dq;
+                col += dr;
+                if (col >= L) {
+                    col -= L;
+                    ++row;
+                }
+            }
+            return;
+        }
         for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
             if (j0 + kH < tw) fetch(tb, j0 + kH);
             reduce_half(ta, j0, 0);

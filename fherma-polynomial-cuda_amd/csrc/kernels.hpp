@@ -53,7 +53,10 @@ void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* pl
 void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
                                 hipStream_t stream);
 void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream);
-// Coefficients [i_begin, i_begin + count), i_begin a multiple of 64.
+// Coefficients [i_begin, i_begin + count), i_begin a multiple of 64.  When
+// reconstruct_writes_once(s), c is only written (each word once), so it may be
+// pinned host memory; otherwise c is read back and must be device memory.
+inline bool reconstruct_writes_once(const Shape& s) { return s.t_words <= 64; }
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
                         uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream);
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream);

@@ -282,11 +282,15 @@ struct pm_plan {
     }
     // First half of a host-vector product.  Per coefficient chunk, the copy
     // workers move a and b (one job) from the caller's pageable vectors into
-    // pinned staging, the H2D goes on a copy stream, and the compute stream
-    // waits on an event before packing and converting that chunk: the host
-    // copy of chunk k+1 overlaps the H2D of chunk k and the conversion of chunk
-    // k-1.  After the transforms and the CRT GEMM, each chunk is reconstructed
-    // and sent D2H on the second copy stream while the next is reconstructed.
+    // pinned staging as [a rows | b rows], one H2D carries both on a copy
+    // stream, and the compute stream waits on an event before packing and
+    // converting that chunk: the host copy of chunk k+1 overlaps the H2D of
+    // chunk k and the conversion of chunk k-1.  After the transforms and the CRT
+    // GEMM, each chunk is reconstructed and reaches pinned memory while the next
+    // is reconstructed: written there by the kernel itself when it stores each
+    // word once (short q), else by a D2H on the second copy stream.  Each DMA
+    // and each hand-off between the compute stream and a copy stream costs some
+    // 10-40 us on this link, which is why both are kept few.
     // Returns with the device busy; multiply_finish collects c.
     // b == nullptr: the b residues come from `bplanes_in` (a resident operand).
     void multiply_begin(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in) {
@@ -302,8 +306,6 @@ struct pm_plan {
         lane->reserve(size_t(s.n) * L);
         hipStream_t stream = lane->stream, copy_in = lane->copy_in, copy_out = lane->copy_out;
         pm::CopyPool* pool = lane->pool.get();
-        uint32_t* pin_a = lane->pin_in;
-        uint32_t* pin_b = lane->pin_in + size_t(s.n) * L;
         pool->set_active(true);  // workers spin between the chunk jobs of this product
         struct Idle {
             pm::CopyPool* pool;
@@ -316,29 +318,41 @@ struct pm_plan {
         for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
             chunks.rows(ch, s.n, r0, r1);
-            if (r1 > r0) {
-                const size_t off = r0 * L, bytes = (r1 - r0) * row_bytes;
-                const pm::CopyPool::Segment segs[2] = {{pin_a + off, a + off, bytes}, {pin_b + off, b + off, bytes}};
-                pool->copy(segs, b ? 2 : 1);
-                hip_check(hipMemcpyAsync(io_a + off, pin_a + off, bytes, hipMemcpyHostToDevice, copy_in), "H2D a");
-                if (b)
-                    hip_check(hipMemcpyAsync(io_b + off, pin_b + off, bytes, hipMemcpyHostToDevice, copy_in),
-                              "H2D b");
+            // the chunk's rows of a, then (two operands) its rows of b, at word
+            // offset nops*r0*L in both staging and io_a
+            const size_t base = nops * r0 * L, rows = r1 - r0;
+            if (rows) {
+                const pm::CopyPool::Segment segs[2] = {{lane->pin_in + base, a + r0 * L, rows * row_bytes},
+                                                       {lane->pin_in + base + rows * L, b ? b + r0 * L : nullptr,
+                                                        rows * row_bytes}};
+                pool->copy(segs, nops);
+                hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
+                                         hipMemcpyHostToDevice, copy_in),
+                          "H2D inputs");
             }
             hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
             hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
             const uint32_t t0 = ch * chunks.per, tc = std::min(chunks.per, s.n_pad / 256 - t0);
             if (tc) {
-                pm::launch_pack(s, io_a, io_b, work.packed, t0, tc, 0, nops, stream);
+                // pack reads coefficient i at x + i*L: offset the bases so rows
+                // r0.. land on the chunk's a and b blocks
+                const uint32_t* xa = io_a + (nops == 2 ? r0 * L : 0);
+                const uint32_t* xb = io_a + r1 * L;
+                pm::launch_pack(s, xa, xb, work.packed, t0, tc, 0, nops, stream);
                 pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
             }
         }
         run_middle(stream, nops, bplanes);
+        const bool direct = pm::reconstruct_writes_once(s);
         for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
             chunks.rows(ch, s.n, r0, r1);
-            pm::launch_reconstruct(s, tables, work.words, work.t_words, io_c, uint32_t(r0), uint32_t(r1 - r0),
-                                   stream);
+            pm::launch_reconstruct(s, tables, work.words, work.t_words, direct ? lane->pin_out : io_c, uint32_t(r0),
+                                   uint32_t(r1 - r0), stream);
+            if (direct) {
+                hip_check(hipEventRecord(lane->out_ready[ch], stream), "hipEventRecord");
+                continue;
+            }
             hip_check(hipEventRecord(lane->reconstructed[ch], stream), "hipEventRecord");
             hip_check(hipStreamWaitEvent(copy_out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
             if (r1 > r0)
@@ -390,8 +404,8 @@ struct pm_plan {
         }
         const pm::Shape& s = host.shape;
         const size_t nl = size_t(s.n) * s.limbs;
-        io_a = alloc<uint32_t>(nl);
-        io_b = alloc<uint32_t>(nl);
+        io_a = alloc<uint32_t>(2 * nl);  // a then b; the host path stages chunks as [a rows | b rows]
+        io_b = io_a + nl;
         io_c = alloc<uint32_t>(nl);
         if (s.trivial) return;
         pm::configure_kernels();
@@ -558,7 +572,7 @@ struct pm_operand {
 inline size_t plan_footprint(const pm::HostPlan& h) {
     const pm::Shape& s = h.shape;
     const size_t nl = size_t(s.n) * s.limbs * sizeof(uint32_t);
-    size_t t = 3 * aligned_bytes(nl);  // io_a, io_b, io_c
+    size_t t = aligned_bytes(2 * nl) + aligned_bytes(nl);  // io_a with io_b, io_c
     if (s.trivial) return t;
     const size_t r = s.r;
     t += aligned_bytes(r * sizeof(pm::PrimeConst)) + aligned_bytes(2 * r * 4) + aligned_bytes(r * 8) +
