@@ -118,3 +118,26 @@ def write_point_dir(root, n, width, limbs, q_words, cases):
         d.mkdir(parents=True, exist_ok=True)
         np.ascontiguousarray(a, dtype="<u4").tofile(d / "a.bin")
         np.ascontiguousarray(b, dtype="<u4").tofile(d / "b.bin")
+
+
+def write_reference_point_dir(root, key="65536/1747", cases=21, seed0=7000):
+    """The reference's competition point `key` (N/W, q from
+    tests/golden/points.json) as a case directory, case i from seed seed0 + i."""
+    import json
+    from pathlib import Path
+
+    points = json.loads((Path(__file__).resolve().parent.parent / "tests" / "golden" / "points.json").read_text())
+    p = points[key]
+    q = int_to_words(int(p["q"], 16), p["L"])
+    write_point_dir(root, p["N"], p["W"], p["L"], q, [operands_for(p["N"], q, seed0 + i) for i in range(cases)])
+
+
+if __name__ == "__main__":
+    import argparse
+
+    parser = argparse.ArgumentParser(description="write a reference competition point as a case directory")
+    parser.add_argument("root")
+    parser.add_argument("--point", default="65536/1747", help="N/W key of tests/golden/points.json")
+    parser.add_argument("--cases", type=int, default=21)
+    args = parser.parse_args()
+    write_reference_point_dir(args.root, args.point, args.cases)
