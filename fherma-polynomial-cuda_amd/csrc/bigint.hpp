@@ -89,6 +89,17 @@ inline Words mul_word(const Words& a, uint32_t w) {
     return out;
 }
 
+// a *= w in place (w != 0 keeps a trimmed).
+inline void mul_word_inplace(Words& a, uint32_t w) {
+    uint64_t carry = 0;
+    for (auto& x : a) {
+        const uint64_t t = uint64_t(x) * w + carry;
+        x = uint32_t(t);
+        carry = t >> 32;
+    }
+    if (carry) a.push_back(uint32_t(carry));
+}
+
 inline Words mul(const Words& a, const Words& b) {
     Words out(a.size() + b.size(), 0);
     for (size_t i = 0; i < a.size(); ++i) {

@@ -4,6 +4,8 @@ closed form computed here with Python integers, and the whole arithmetic of
 the GPU path is replayed with numpy from the exported tables alone and
 compared limb-for-limb with the oracle."""
 import json
+import shutil
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -152,6 +154,44 @@ def test_crt_tables(polymul, oracle, workloads, n, width, kind):
         h = q.bit_length() - 1
         assert s["mu"] == (1 << (h + 63)) // q
         assert s["s_shift"] == max(h - 16, 0) and s["k_shift"] == h - s["s_shift"] + 63
+
+
+def test_crt_tables_when_a_prime_divides_q(polymul, oracle, workloads):
+    """q = p_0 * (odd cofactor): M_0 takes the direct-product branch, every
+    other M_k the exact-quotient one."""
+    n, limbs = 1024, 8
+    probe = workloads.words_to_int(workloads.random_modulus(limbs, np.random.default_rng(5)))
+    with _plan(polymul, oracle, n, 32 * limbs, probe) as plan:
+        p0 = int(plan.primes()[0])
+    cofactor = workloads.words_to_int(workloads.random_modulus(limbs - 1, np.random.default_rng(6)))
+    q = p0 * cofactor
+    with _plan(polymul, oracle, n, 32 * limbs, q) as plan:
+        primes = [int(p) for p in plan.primes()]
+        assert primes[0] == p0
+        r = plan.shape()["r"]
+        product = 1
+        for p in primes:
+            product *= p
+        m = plan.table(polymul.TABLE_M_MOD_Q, np.uint32).reshape(r, limbs)
+        for k, p in enumerate(primes):
+            assert oracle.words_to_int(m[k]) == (product // p) % q, k
+        assert oracle.words_to_int(plan.table(polymul.TABLE_NEG_P_MOD_Q, np.uint32)) == (-product) % q
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ to build the walk check")
+def test_prime_walk_matches_the_plain_walk(tmp_path):
+    """The walk's interval bounds and parallel primality tests take exactly the
+    primes of the plain exact walk, also where the bounds straddle the bound
+    (tests/walk_check.cpp)."""
+    root = Path(__file__).resolve().parent.parent
+    csrc = root / "fherma-polynomial-cuda_amd" / "csrc"
+    exe = tmp_path / "walk_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(root / "include"), "-I", str(csrc),
+                    str(root / "tests" / "walk_check.cpp"), str(csrc / "plan.cpp"), "-o", str(exe), "-lpthread"],
+                   check=True, capture_output=True, timeout=300)
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stdout[-2000:]
+    assert proc.stdout.startswith("ok ")
 
 
 def _redc_signed(value, p):
