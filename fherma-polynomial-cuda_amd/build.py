@@ -4,8 +4,9 @@
   libfherma_amd.so      fherma_shim.cpp linked to the above (include/fherma.h)
   solution              driver_main.cpp: the main.cpp-compatible case runner
 
-Every HIP translation unit is compiled for gfx950 only, with packed-FP32
-instructions disabled (the kernels are integer-only; the flag is a safety net).
+Every translation unit is compiled for gfx950 only (hipcc treats the .cpp
+sources as HIP too), with packed-FP32 instructions disabled (the kernels are
+integer-only; the flag is a safety net).
 """
 import os
 import subprocess
@@ -43,18 +44,18 @@ def build(verbose=False):
     objects = []
     for src in ("kernels.hip", "api.cpp", "serve.cpp", "plan.cpp"):
         obj = obj_dir / (src.rsplit(".", 1)[0] + ".o")
-        flags = COMMON + (DEVICE if src.endswith(".hip") else [])
+        flags = COMMON + DEVICE  # .cpp sources hold no kernels; gfx950 keeps hipcc off its default target
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC] + flags + lang + ["-c", CSRC / src, "-o", obj], verbose)
         objects.append(obj)
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH] + objects + ["-o", LIB], verbose)
     shim = CSRC / "fherma_shim.cpp"
     if shim.is_file():
-        _run([HIPCC] + COMMON + ["-shared", shim, "-L", PKG, "-lpolymul_amd",
+        _run([HIPCC] + COMMON + DEVICE + ["-shared", shim, "-L", PKG, "-lpolymul_amd",
                                  "-Wl,-rpath,$ORIGIN", "-o", FHERMA_LIB], verbose)
         driver = CSRC / "driver_main.cpp"
         if driver.is_file():
-            _run([HIPCC] + COMMON + [driver, "-L", PKG, "-lfherma_amd", "-lpolymul_amd",
+            _run([HIPCC] + COMMON + DEVICE + [driver, "-L", PKG, "-lfherma_amd", "-lpolymul_amd",
                                      "-Wl,-rpath,$ORIGIN", "-o", SOLUTION], verbose)
     return LIB
 

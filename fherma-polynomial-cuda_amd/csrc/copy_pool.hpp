@@ -11,13 +11,16 @@
 // which is what bounds these copies.
 //
 // A product hands the pool one job per chunk (both operands' slices at once).
-// While a product is in flight (set_active(true)) idle workers spin on the job
-// counter, so a hand-off costs about a microsecond instead of a condition-
-// variable wake-up; otherwise they sleep.
+// While a product is in flight (set_active(true)) an idle worker spins on the
+// job counter for up to 300 us, so a hand-off between closely spaced chunks
+// costs about a microsecond instead of a condition-variable wake-up; after
+// that, or when no product is in flight, it sleeps (several lanes' workers
+// would otherwise burn the cores that plan construction needs).
 #pragma once
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstddef>
 #include <cstdint>
@@ -85,13 +88,16 @@ public:
 
     unsigned threads() const { return parts_; }
 
-    // Spin (true) or sleep (false) while idle.  Not reentrant with copy().
+    // A product starts (true: wake sleeping workers to spin for the first
+    // chunk) or ends (false: idle workers sleep).  Not reentrant with copy().
     void set_active(bool on) {
-        {
+        active_.store(on, std::memory_order_seq_cst);
+        if (!on) return;
+        pokes_.fetch_add(1, std::memory_order_seq_cst);
+        if (sleepers_.load(std::memory_order_seq_cst) != 0) {
             std::lock_guard<std::mutex> lock(mutex_);
-            active_.store(on, std::memory_order_release);
+            wake_.notify_all();
         }
-        if (on) wake_.notify_all();
     }
 
     void copy(void* dst, const void* src, size_t bytes) {
@@ -111,13 +117,12 @@ public:
         }
         segs_.assign(segs, segs + count);
         pending_.store(parts_ - 1, std::memory_order_relaxed);
-        if (active_.load(std::memory_order_acquire)) {
-            generation_.fetch_add(1, std::memory_order_release);  // spinning workers see it
-        } else {
-            {
-                std::lock_guard<std::mutex> lock(mutex_);
-                generation_.fetch_add(1, std::memory_order_release);
-            }
+        // Publish, then wake sleepers.  A worker registers as a sleeper under
+        // the mutex before it re-checks the counter, so either it sees this
+        // job or this thread sees it registered (both sequentially consistent).
+        generation_.fetch_add(1, std::memory_order_seq_cst);
+        if (sleepers_.load(std::memory_order_seq_cst) != 0) {
+            std::lock_guard<std::mutex> lock(mutex_);
             wake_.notify_all();
         }
         share(0);
@@ -143,18 +148,23 @@ private:
         // this thread got here must still be taken
         size_t seen = 0;
         for (;;) {
-            // spin while the pool is active, sleep otherwise
-            while (generation_.load(std::memory_order_acquire) == seen && active_.load(std::memory_order_acquire) &&
-                   !stop_.load(std::memory_order_acquire))
-                pause();
-            if (generation_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_acquire)) {
+            auto spin_until = std::chrono::steady_clock::now() + std::chrono::microseconds(300);
+            unsigned polls = 0;
+            while (generation_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_acquire)) {
+                if (active_.load(std::memory_order_acquire) &&
+                    ((++polls & 255) || std::chrono::steady_clock::now() < spin_until)) {
+                    pause();
+                    continue;
+                }
+                const size_t poke = pokes_.load(std::memory_order_seq_cst);
                 std::unique_lock<std::mutex> lock(mutex_);
+                sleepers_.fetch_add(1, std::memory_order_seq_cst);
                 wake_.wait(lock, [&] {
-                    return generation_.load(std::memory_order_acquire) != seen ||
-                           active_.load(std::memory_order_acquire) || stop_.load(std::memory_order_acquire);
+                    return generation_.load(std::memory_order_seq_cst) != seen ||
+                           stop_.load(std::memory_order_acquire) || pokes_.load(std::memory_order_seq_cst) != poke;
                 });
-                if (generation_.load(std::memory_order_acquire) == seen && !stop_.load(std::memory_order_acquire))
-                    continue;  // woken to spin
+                sleepers_.fetch_sub(1, std::memory_order_seq_cst);
+                spin_until = std::chrono::steady_clock::now() + std::chrono::microseconds(300);
             }
             if (stop_.load(std::memory_order_acquire)) return;
             seen = generation_.load(std::memory_order_acquire);
@@ -170,6 +180,8 @@ private:
     std::atomic<size_t> generation_{0};
     std::atomic<bool> active_{false};
     std::atomic<bool> stop_{false};
+    std::atomic<unsigned> sleepers_{0};
+    std::atomic<size_t> pokes_{0};
     std::vector<Segment> segs_;
     std::atomic<unsigned> pending_{0};
 };

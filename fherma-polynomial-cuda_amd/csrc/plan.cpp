@@ -77,13 +77,22 @@ std::vector<int8_t> signed_digits(const uint32_t* words, size_t nwords, size_t c
     return out;
 }
 
-// fn(begin, end) over [0, count) split across up to 16 host threads (the
-// calling thread takes the first share); serial below `min_share` per thread.
-// fn must not throw.
-template <class F>
-void parallel_range(size_t count, size_t min_share, F&& fn) {
+thread_local unsigned t_plan_threads = 0;  // set_plan_threads
+
+// Threads plan construction may use from this thread: the machine's, at most
+// 16, and at most the caller's set_plan_threads share.
+unsigned plan_threads() {
     unsigned threads = std::thread::hardware_concurrency();
     threads = threads ? std::min(threads, 16u) : 1u;
+    return t_plan_threads ? std::min(threads, t_plan_threads) : threads;
+}
+
+// fn(begin, end) over [0, count) split across up to plan_threads() host
+// threads (the calling thread takes the first share); serial below
+// `min_share` per thread.  fn must not throw.
+template <class F>
+void parallel_range(size_t count, size_t min_share, F&& fn) {
+    unsigned threads = plan_threads();
     threads = unsigned(std::max<size_t>(1, std::min<size_t>(threads, count / std::max<size_t>(1, min_share))));
     std::vector<std::thread> pool;
     for (unsigned t = 1; t < threads; ++t)
@@ -195,6 +204,8 @@ uint32_t newton_inverse(uint32_t p) {
 }
 
 }  // namespace
+
+void set_plan_threads(unsigned threads) { t_plan_threads = threads; }
 
 bool is_prime_u32(uint32_t p) {
     if (p < 2) return false;
@@ -421,7 +432,7 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
 
     // Per-prime constants (and, for host tables, each prime's rows): independent
     // per prime, spread over host threads.
-    std::vector<std::string> prime_errors(16);
+    std::vector<std::string> prime_errors(plan_threads());
     std::atomic<unsigned> prime_part{0};
     parallel_range(r, host_tables ? 1 : 64, [&](size_t k0, size_t k1) {
         const unsigned part = prime_part.fetch_add(1);
@@ -520,8 +531,7 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
     std::vector<uint32_t> qw(L, 0), pqw(L, 0);
     for (size_t i = 0; i < q.size() && i < L; ++i) qw[i] = q[i];
     for (size_t i = 0; i < pq.size() && i < L; ++i) pqw[i] = pq[i];
-    unsigned threads = std::thread::hardware_concurrency();
-    threads = threads ? std::min(threads, 16u) : 1u;
+    const unsigned threads = plan_threads();
     std::vector<std::vector<uint64_t>> partial(threads, std::vector<uint64_t>(L + 1, 0));
     std::vector<std::string> errors(threads);
     std::atomic<unsigned> next_part{0};

@@ -81,6 +81,10 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
 // floor((2^31-2)/step)*step + 1, primes only, while prod(p) <= bound, failing
 // below 2^30.  step = 2N reproduces solve.cu:905-913 exactly.
 bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes);
+// Host threads that plan construction started from the calling thread may use
+// (0: as many as the machine has, at most 16).  A server's workers each take a
+// share, so that concurrent plan builds do not oversubscribe the host.
+void set_plan_threads(unsigned threads);
 
 bool is_prime_u32(uint32_t p);
 

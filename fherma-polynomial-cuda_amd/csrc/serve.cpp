@@ -3,16 +3,21 @@
 // No counterpart in the reference, whose fherma_run multiplies one pair at a
 // time (candidates/ordered-streaming/solve.cu:1226-1271).
 //
-// A request's life: its worker builds the host part of its plan (prime walk,
-// M_k; tables come later on the device), reserves the plan's exact device
-// footprint in the arena -- waiting there while other requests hold the memory,
-// which is the admission rule -- then builds the device tables, runs the
-// host-vector product on the lane's streams, and gives the block back.  Requests
-// on different lanes overlap on the device (transfers of one with kernels of
-// another, small products side by side); the lane's streams, events, pinned
-// staging and copy workers are made once and reused by every request it runs.
+// A request's life: a planner thread builds the host part of its plan (prime
+// walk, M_k; tables come later on the device) as soon as it arrives, up to
+// 2 * lanes requests ahead of the workers, so plan construction overlaps the
+// products in flight.  A worker then takes the oldest request, reserves its
+// plan's exact device footprint in the arena -- waiting there while other
+// requests hold the memory, which is the admission rule -- builds the device
+// tables, runs the host-vector product on the lane's streams, and gives the
+// block back.  Requests on different lanes overlap on the device (transfers of
+// one with kernels of another, small products side by side); the lane's
+// streams, events, pinned staging and copy workers are made once and reused by
+// every request it runs.
+#include <algorithm>
 #include <chrono>
 #include <deque>
+#include <exception>
 #include <unordered_map>
 
 #include "polymul.h"
@@ -32,7 +37,13 @@ struct Job {
     pm_status status = PM_OK;
     std::string error;
     bool finished = false;
+    // built ahead by a planner (guarded by the server's mutex until planned)
+    std::unique_ptr<pm::HostPlan> host;
+    std::exception_ptr plan_error;
+    bool planned = false, taken = false;
 };
+
+constexpr unsigned kPlanners = 2;
 
 }  // namespace
 
@@ -40,20 +51,61 @@ struct pm_server {
     int device_id = 0;
     std::unique_ptr<pm::DeviceArena> arena;
     std::vector<std::unique_ptr<pm::Lane>> lanes;
-    std::vector<std::thread> workers;
+    std::vector<std::thread> workers, planners;
     std::mutex mutex;
-    std::condition_variable work, done;
-    std::deque<Job*> queue;
+    std::condition_variable work, done, plan_work, planned;
+    std::deque<Job*> queue, plan_queue;
+    size_t planned_ahead = 0, lookahead = 2;  // plans built for requests no worker has taken
     std::unordered_map<uint64_t, std::unique_ptr<Job>> jobs;
     uint64_t next_ticket = 1;
     bool stopping = false;
 
+    void planner() {
+        const unsigned hw = std::thread::hardware_concurrency();
+        pm::set_plan_threads(std::max(2u, (hw ? hw : 4u) / kPlanners));
+        for (;;) {
+            Job* job = nullptr;
+            {
+                std::unique_lock<std::mutex> lock(mutex);
+                plan_work.wait(lock, [&] {
+                    return (!plan_queue.empty() && planned_ahead < lookahead) || (stopping && plan_queue.empty());
+                });
+                if (plan_queue.empty()) return;  // stopping, and every request has its plan
+                job = plan_queue.front();
+                plan_queue.pop_front();
+            }
+            std::unique_ptr<pm::HostPlan> host;
+            std::exception_ptr error;
+            try {
+                const pm_request& rq = job->request;
+                if (!rq.q || !rq.a || !rq.b || !rq.c) throw std::invalid_argument("null request buffer");
+                host = std::make_unique<pm::HostPlan>(
+                    pm::build_plan(rq.n, rq.width, rq.limbs, rq.q, pm::Tables::kDevice));
+            } catch (...) {
+                error = std::current_exception();
+            }
+            {
+                std::lock_guard<std::mutex> lock(mutex);
+                job->host = std::move(host);
+                job->plan_error = error;
+                job->planned = true;
+                if (!job->taken) ++planned_ahead;
+            }
+            planned.notify_all();
+        }
+    }
+
     void run(pm::Lane& lane, Job& job) {
+        {
+            std::unique_lock<std::mutex> lock(mutex);
+            planned.wait(lock, [&] { return job.planned; });
+        }
         const pm_request& rq = job.request;
         job.status = guarded([&] {
-            if (!rq.q || !rq.a || !rq.b || !rq.c) throw std::invalid_argument("null request buffer");
+            if (job.plan_error) std::rethrow_exception(job.plan_error);
             auto plan = std::make_unique<pm_plan>();
-            plan->host = pm::build_plan(rq.n, rq.width, rq.limbs, rq.q, pm::Tables::kDevice);
+            plan->host = std::move(*job.host);
+            job.host.reset();
             const size_t need = plan_footprint(plan->host);
             char* block = arena->acquire(need);
             job.admitted = Clock::now();
@@ -91,7 +143,12 @@ struct pm_server {
                 if (queue.empty()) return;  // stopping, and every request has been taken
                 job = queue.front();
                 queue.pop_front();
+                if (job->planned)
+                    --planned_ahead;
+                else
+                    job->taken = true;
             }
+            plan_work.notify_all();  // a place in the look-ahead may have freed
             run(lane, *job);
             {
                 std::lock_guard<std::mutex> lock(mutex);
@@ -107,9 +164,13 @@ struct pm_server {
             stopping = true;
         }
         work.notify_all();
-        for (auto& w : workers)
+        plan_work.notify_all();
+        for (auto& w : workers)  // they finish every queued request (planners still run)
             if (w.joinable()) w.join();
         workers.clear();
+        for (auto& p : planners)
+            if (p.joinable()) p.join();
+        planners.clear();
     }
 
     ~pm_server() { stop(); }
@@ -130,7 +191,9 @@ pm_status server_create(uint64_t budget, uint32_t lanes, pm_server** out) {
         sv->arena = std::make_unique<pm::DeviceArena>(size_t(budget));
         const unsigned copy_threads = std::max(2u, pm::Lane::default_copy_threads() / lanes);
         for (uint32_t i = 0; i < lanes; ++i) sv->lanes.push_back(std::make_unique<pm::Lane>(copy_threads));
+        sv->lookahead = 2 * size_t(lanes);
         for (uint32_t i = 0; i < lanes; ++i) sv->workers.emplace_back([s = sv.get(), i] { s->worker(i); });
+        for (unsigned i = 0; i < kPlanners; ++i) sv->planners.emplace_back([s = sv.get()] { s->planner(); });
         *out = sv.release();
     });
 }
@@ -154,8 +217,10 @@ pm_status pm_server_submit(pm_server* server, const pm_request* request, uint64_
             if (server->stopping) throw std::invalid_argument("server is shutting down");
             *ticket = server->next_ticket++;
             server->queue.push_back(job.get());
+            server->plan_queue.push_back(job.get());
             server->jobs.emplace(*ticket, std::move(job));
         }
+        server->plan_work.notify_one();
         server->work.notify_one();
     });
 }
@@ -201,7 +266,15 @@ pm_status pm_multiply_batch(const pm_request* requests, size_t count, uint64_t d
     std::vector<uint64_t> tickets(count, 0);
     pm_status first = PM_OK;
     std::string first_error;
-    for (size_t i = 0; i < count; ++i) {
+    // Longest first: the whole batch is known, so the largest products start
+    // while the small ones fill the lanes at the end, instead of one large
+    // product running alone after the rest (a product's work grows about as
+    // N * L^2, the GEMMs').
+    std::vector<size_t> order(count);
+    for (size_t i = 0; i < count; ++i) order[i] = i;
+    auto cost = [&](size_t i) { return double(requests[i].n) * requests[i].limbs * requests[i].limbs; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(x) > cost(y); });
+    for (size_t i : order) {
         pm_status s = pm_server_submit(server, &requests[i], &tickets[i]);
         if (s != PM_OK) {
             tickets[i] = 0;

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "copy_pool.hpp"
+#include "pinned_cache.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 #include "polymul.h"
@@ -89,15 +90,20 @@ public:
 
     // Pinned staging for products of up to `words` words per operand:
     // [2][words] in (a then b), [words] out.
+    // The blocks come from (and go back to) the process-wide PinnedCache.
     void reserve(size_t words) {
         if (words <= pin_words) return;
         synchronize();
         free_pinned();
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pin_in), 2 * words * sizeof(uint32_t), hipHostMallocDefault),
-                  "pin input staging");
-        hip_check(hipHostMalloc(reinterpret_cast<void**>(&pin_out), words * sizeof(uint32_t), hipHostMallocDefault),
-                  "pin output staging");
-        pin_words = words;
+        PinnedCache& cache = PinnedCache::instance();
+        pin_in = static_cast<uint32_t*>(cache.take(2 * words * sizeof(uint32_t), &pin_in_bytes));
+        try {
+            pin_out = static_cast<uint32_t*>(cache.take(words * sizeof(uint32_t), &pin_out_bytes));
+        } catch (...) {
+            free_pinned();
+            throw;
+        }
+        pin_words = std::min(pin_in_bytes / (2 * sizeof(uint32_t)), pin_out_bytes / sizeof(uint32_t));
     }
 
     hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
@@ -109,11 +115,14 @@ public:
     size_t pin_words = 0;
 
 private:
+    size_t pin_in_bytes = 0, pin_out_bytes = 0;
+
     void free_pinned() {
-        if (pin_in) (void)hipHostFree(pin_in);
-        if (pin_out) (void)hipHostFree(pin_out);
+        PinnedCache& cache = PinnedCache::instance();
+        if (pin_in) cache.give(pin_in, pin_in_bytes);
+        if (pin_out) cache.give(pin_out, pin_out_bytes);
         pin_in = pin_out = nullptr;
-        pin_words = 0;
+        pin_in_bytes = pin_out_bytes = pin_words = 0;
     }
     void release() {
         synchronize();

@@ -179,8 +179,10 @@ pm_status pm_server_wait(pm_server* server, uint64_t ticket, pm_request_times* t
 /* Completes every queued request, then releases the lanes and the arena. */
 void pm_server_destroy(pm_server* server);
 
-/* Every request through a temporary server; statuses (optional, [count]) gets
- * each request's status; the return value is the first failure, or PM_OK. */
+/* Every request through a temporary server, submitted largest first (by
+ * N * L^2) so that small products fill the lanes at the end; statuses
+ * (optional, [count]) gets each request's status; the return value is the first
+ * failure (in submission order), or PM_OK. */
 pm_status pm_multiply_batch(const pm_request* requests, size_t count, uint64_t device_budget_bytes, uint32_t lanes,
                             pm_status* statuses);
 
