@@ -118,21 +118,33 @@ def plain_products(polymul, products):
     return outs, times
 
 
-def batched_leg(polymul, workloads, count, lanes):
+def batched_leg(polymul, workloads, count, lanes, passes=3):
     """64 products: N in {2^12..2^16}, L log-uniform in [16, 1747] limbs, own q
-    each; device footprint capped at 60 GB; plain path vs pm_multiply_batch."""
+    each; device footprint capped at 60 GB; plain path vs pm_multiply_batch.
+    Each form first runs once untimed (pinned staging filled for the first
+    time, first launches at each shape: both forms pay these once per process),
+    then `passes` timed passes over the same 64 products; products/s is from
+    the median pass.  The untimed first passes are reported as *_cold."""
     products = make_products(workloads, count, 0x5EED0004, [4096, 8192, 16384, 32768, 65536], (16, 1747))
-    t0 = time.perf_counter()
-    plain, _ = plain_products(polymul, products)
-    plain_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    batched = polymul.multiply_batch(products, 60 * GiB, lanes=lanes)
-    batched_s = time.perf_counter() - t0
-    exact = all(np.array_equal(x, y) for x, y in zip(plain, batched))
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        out = fn()
+        return time.perf_counter() - t0, out
+
+    plain_cold, (plain, _) = timed(lambda: plain_products(polymul, products))
+    plain_s = float(np.median([timed(lambda: plain_products(polymul, products))[0] for _ in range(passes)]))
+    batched_cold, batched = timed(lambda: polymul.multiply_batch(products, 60 * GiB, lanes=lanes))
+    runs = [timed(lambda: polymul.multiply_batch(products, 60 * GiB, lanes=lanes)) for _ in range(passes)]
+    batched_s = float(np.median([s for s, _ in runs]))
+    exact = all(np.array_equal(x, y) for x, y in zip(plain, batched)) and all(
+        np.array_equal(x, y) for x, y in zip(plain, runs[-1][1]))
     words = sum(n * limbs for n, limbs, *_ in products)
-    return {"products": count, "lanes": lanes, "device_budget_gib": 60,
+    return {"products": count, "lanes": lanes, "device_budget_gib": 60, "timed_passes": passes,
             "plain_products_per_s": round(count / plain_s, 2), "batched_products_per_s": round(count / batched_s, 2),
             "batched_vs_plain": round(plain_s / batched_s, 2), "exact_vs_plain": exact,
+            "plain_cold_products_per_s": round(count / plain_cold, 2),
+            "batched_cold_products_per_s": round(count / batched_cold, 2),
             "host_link_gb": round(12 * words / 1e9, 2)}, products
 
 
