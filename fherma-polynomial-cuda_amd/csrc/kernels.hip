@@ -677,13 +677,34 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
     for (int s = 0; s < 8; ++s) {
         constexpr int n1i = int(N1);
         const int s1 = s % n1i, blk0 = s - s1;  // block start inside the 8 values
+        // Products are summed two at a time before one REDC: a*b + c*d < 2p^2 <
+        // p*2^32 (p < 2^31), inside REDC's range, so a pair costs one reduction
+        // (two multiplies) less and lands on the same residue.  (32-bit
+        // multiplies are quarter rate and bound this kernel.)
         uint32_t lo = 0, hi = 0;
+        uint64_t acc_lo = 0, acc_hi = 0;
+        int pending_lo = 0, pending_hi = 0;
 #pragma unroll
         for (int u = 0; u < n1i; ++u) {
-            const uint32_t prod = mont(xa[blk0 + u], xb[blk0 + (s1 >= u ? s1 - u : s1 + n1i - u)], p, pinv);
-            if (u <= s1) lo = add_mod(lo, prod, p);
-            else hi = add_mod(hi, prod, p);
+            const uint64_t prod = uint64_t(xa[blk0 + u]) * xb[blk0 + (s1 >= u ? s1 - u : s1 + n1i - u)];
+            if (u <= s1) {
+                acc_lo += prod;
+                if (++pending_lo == 2) {
+                    lo = add_mod(lo, redc(acc_lo, p, pinv), p);
+                    acc_lo = 0;
+                    pending_lo = 0;
+                }
+            } else {
+                acc_hi += prod;
+                if (++pending_hi == 2) {
+                    hi = add_mod(hi, redc(acc_hi, p, pinv), p);
+                    acc_hi = 0;
+                    pending_hi = 0;
+                }
+            }
         }
+        if (pending_lo) lo = add_mod(lo, redc(acc_lo, p, pinv), p);
+        if (pending_hi) hi = add_mod(hi, redc(acc_hi, p, pinv), p);
         uint32_t value = lo;
         if (n1 > 1) {
             const uint32_t gb = (row * kRowC + 8 * t + blk0) / n1;  // global block index
