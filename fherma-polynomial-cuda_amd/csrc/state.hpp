@@ -18,6 +18,7 @@
 #include <thread>
 #include <vector>
 
+#include "arena.hpp"
 #include "copy_pool.hpp"
 #include "pinned_cache.hpp"
 #include "kernels.hpp"
@@ -37,9 +38,7 @@ inline void hip_check(hipError_t e, const char* where) {
     if (e != hipSuccess) throw DeviceError(std::string(where) + ": " + hipGetErrorString(e));
 }
 
-// Device allocations are rounded to 256 bytes (GEMM tiles load 16-byte vectors;
-// every buffer starts on a 256-byte boundary).
-inline size_t aligned_bytes(size_t bytes) { return ((bytes ? bytes : 1) + 255) & ~size_t(255); }
+using pm::aligned_bytes;  // arena.hpp: device allocations in 256-byte units
 
 // Events bracket the 7 stage launches; times are the 7 intervals plus the total.
 constexpr int kEvents = 8;
@@ -136,133 +135,6 @@ private:
             if (st) (void)hipStreamDestroy(st);
         stream = copy_in = copy_out = nullptr;
     }
-};
-
-// Device memory of a server: at most `budget` bytes, taken from hipMalloc in
-// chunks as requests need them (a chunk holds at least 4 GiB or the request,
-// never past the budget) and handed out first-fit within the chunks, with
-// coalescing.  acquire() blocks while no chunk has room and a new chunk would
-// pass the budget: that is the admission rule.  If the device refuses a new
-// chunk, chunks with nothing in use are given back and the allocation is
-// retried; a request that still cannot be placed while nothing of the arena's
-// is in use fails with std::bad_alloc.  (Reserving the whole budget up front
-// cost from a few ms to 0.7 s per server in hipMalloc, and up to 0.8 s more in
-// hipFree, on the MI355X.)  The chunks are freed with the arena.
-class DeviceArena {
-public:
-    explicit DeviceArena(size_t budget) : budget_(budget & ~size_t(255)) {}
-    ~DeviceArena() {
-        for (Chunk& c : chunks_) (void)hipFree(c.base);
-    }
-    DeviceArena(const DeviceArena&) = delete;
-    DeviceArena& operator=(const DeviceArena&) = delete;
-
-    size_t capacity() const { return budget_; }
-
-    char* acquire(size_t bytes) {
-        bytes = aligned_bytes(bytes);
-        if (bytes > budget_) throw std::bad_alloc();
-        std::unique_lock<std::mutex> lock(mutex_);
-        for (;;) {
-            if (char* p = carve(bytes)) return p;
-            if (allocated_ + bytes <= budget_ && grow(bytes)) continue;
-            if (used_.empty()) throw std::bad_alloc();  // nothing will be released to make room
-            released_.wait(lock);
-        }
-    }
-
-    void release(char* p) {
-        {
-            std::lock_guard<std::mutex> lock(mutex_);
-            auto it = used_.find(p);
-            if (it == used_.end()) return;
-            Chunk& c = chunks_[it->second.chunk];
-            size_t off = size_t(p - c.base), size = it->second.bytes;
-            used_.erase(it);
-            --c.in_use;
-            auto next = c.free.lower_bound(off);
-            if (next != c.free.end() && next->first == off + size) {
-                size += next->second;
-                next = c.free.erase(next);
-            }
-            if (next != c.free.begin()) {
-                auto prev = std::prev(next);
-                if (prev->first + prev->second == off) {
-                    prev->second += size;
-                    size = 0;
-                }
-            }
-            if (size) c.free[off] = size;
-        }
-        released_.notify_all();
-    }
-
-private:
-    struct Chunk {
-        char* base;
-        size_t size;
-        size_t in_use = 0;
-        std::map<size_t, size_t> free;  // offset -> size
-    };
-    struct Use {
-        size_t chunk, bytes;
-    };
-
-    char* carve(size_t bytes) {  // first fit; caller holds the mutex
-        for (size_t ci = 0; ci < chunks_.size(); ++ci) {
-            Chunk& c = chunks_[ci];
-            for (auto it = c.free.begin(); it != c.free.end(); ++it) {
-                if (it->second < bytes) continue;
-                const size_t off = it->first, size = it->second;
-                c.free.erase(it);
-                if (size > bytes) c.free[off + bytes] = size - bytes;
-                ++c.in_use;
-                used_[c.base + off] = Use{ci, bytes};
-                return c.base + off;
-            }
-        }
-        return nullptr;
-    }
-
-    bool grow(size_t bytes) {  // caller holds the mutex
-        const size_t size = std::min(budget_ - allocated_, std::max(bytes, size_t(4) << 30));
-        for (int attempt = 0; attempt < 2; ++attempt) {
-            char* base = nullptr;
-            const hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), size);
-            if (e == hipSuccess) {
-                chunks_.push_back(Chunk{base, size});
-                chunks_.back().free[0] = size;
-                allocated_ += size;
-                return true;
-            }
-            (void)hipGetLastError();  // clear the sticky error of a refused allocation
-            if (e != hipErrorOutOfMemory) hip_check(e, "arena hipMalloc");
-            if (attempt == 0 && !drop_idle_chunks()) break;
-        }
-        return false;
-    }
-
-    bool drop_idle_chunks() {  // frees chunks with nothing in use; caller holds the mutex
-        bool dropped = false;
-        for (size_t ci = chunks_.size(); ci-- > 0;) {
-            if (chunks_[ci].in_use) continue;
-            (void)hipFree(chunks_[ci].base);
-            allocated_ -= chunks_[ci].size;
-            chunks_.erase(chunks_.begin() + ci);
-            dropped = true;
-        }
-        if (dropped)  // chunk indices moved: renumber the blocks in use
-            for (auto& u : used_)
-                for (size_t ci = 0; ci < chunks_.size(); ++ci)
-                    if (u.first >= chunks_[ci].base && u.first < chunks_[ci].base + chunks_[ci].size) u.second.chunk = ci;
-        return dropped;
-    }
-
-    size_t budget_, allocated_ = 0;
-    std::mutex mutex_;
-    std::condition_variable released_;
-    std::vector<Chunk> chunks_;
-    std::map<char*, Use> used_;
 };
 
 }  // namespace pm

@@ -19,3 +19,20 @@ def test_copy_pool_under_stress(tmp_path):
     proc = subprocess.run([str(exe), "1500"], capture_output=True, text=True, timeout=600)
     assert proc.returncode == 0, proc.stdout[-2000:]
     assert proc.stdout.startswith("ok ")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not Path("/opt/rocm/include/hip/hip_runtime.h").is_file(),
+                    reason="needs g++ and the HIP headers to build the arena check")
+def test_device_arena_with_a_fake_device(tmp_path):
+    """The server's device arena (chunks on demand, admission, coalescing, idle
+    chunks given back, refusals) driven by a fake allocator on the CPU
+    (tests/arena_check.cpp)."""
+    root = Path(__file__).resolve().parent.parent
+    csrc = root / "fherma-polynomial-cuda_amd" / "csrc"
+    exe = tmp_path / "arena_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-I", str(csrc),
+                    str(root / "tests" / "arena_check.cpp"), "-o", str(exe), "-lpthread"],
+                   check=True, capture_output=True, timeout=300)
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stdout[-2000:] + proc.stderr[-2000:]
+    assert proc.stdout.strip() == "ok"
