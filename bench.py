@@ -293,7 +293,7 @@ def main():
     polymul = _load("polymul")
     workloads = _load("workloads")
     t_gen = time.perf_counter()
-    n, width, limbs, q, a, b = workloads.baseline_case("large")
+    n, width, limbs, q, a, b = workloads.seeded_case("large")
     gen_s = time.perf_counter() - t_gen
     t_plan = time.perf_counter()
     plan = polymul.Plan(n, width, limbs, q)

@@ -90,8 +90,7 @@ def load_library(path=None):
         return _lib
     path = Path(path) if path else LIBRARY_PATH
     if not path.is_file():
-        raise FileNotFoundError("%s is not built; run `python __graft_entry__.py` or "
-                                "`python fherma-polynomial-cuda_amd/build.py`" % path)
+        raise FileNotFoundError("%s is not built; run `python fherma-polynomial-cuda_amd/build.py`" % path)
     lib = ctypes.CDLL(str(path))
     u32p = ctypes.POINTER(ctypes.c_uint32)
     vp = ctypes.c_void_p

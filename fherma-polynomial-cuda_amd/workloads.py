@@ -78,7 +78,7 @@ def q_minus_one(q_words):
     return q
 
 
-def baseline_case(name, seed=None):
+def seeded_case(name, seed=None):
     """(n, width_bits, limbs, q_words, a, b) for one of CONFIGS."""
     cfg = CONFIGS[name]
     n, limbs = cfg["n"], cfg["limbs"]

@@ -50,7 +50,7 @@ def write_small():
     spec = importlib.util.spec_from_file_location("workloads", ROOT / "fherma-polynomial-cuda_amd" / "workloads.py")
     workloads = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(workloads)
-    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    n, width, limbs, q, a, b = workloads.seeded_case("small")
     qi = reference.words_to_int(q)
     cases = {"random": (a, b)}
     z = np.zeros((n, limbs), dtype=np.uint32)

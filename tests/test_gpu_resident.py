@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("name,count", [("small", 4), ("mid", 2)])
 def test_resident_operand_matches_plain_product(polymul, workloads, oracle, name, count):
     print("[gpu-case] resident", name, flush=True)
-    n, width, limbs, q, a, b = workloads.baseline_case(name)
+    n, width, limbs, q, a, b = workloads.seeded_case(name)
     with polymul.Plan(n, width, limbs, q) as plan:
         with plan.operand(b) as fixed:
             for seed in range(count):
@@ -23,7 +23,7 @@ def test_resident_operand_matches_plain_product(polymul, workloads, oracle, name
 
 def test_resident_operand_device_entry(polymul, workloads, device_memory):
     print("[gpu-case] resident device", flush=True)
-    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    n, width, limbs, q, a, b = workloads.seeded_case("small")
     with polymul.Plan(n, width, limbs, q) as plan:
         expect = plan.multiply(a, b)
         with plan.operand(b) as fixed:
@@ -39,7 +39,7 @@ def test_resident_operand_device_entry(polymul, workloads, device_memory):
 
 
 def test_operand_of_another_plan_is_rejected(polymul, workloads):
-    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    n, width, limbs, q, a, b = workloads.seeded_case("small")
     with polymul.Plan(n, width, limbs, q) as p1, polymul.Plan(n, width, limbs, q) as p2:
         with p1.operand(b) as fixed:
             with pytest.raises(polymul.PolymulError) as err:
@@ -51,7 +51,7 @@ def test_operand_of_another_plan_is_rejected(polymul, workloads):
 def test_device_entry_matches_host_entry(polymul, workloads, oracle, device_memory, name):
     """pm_multiply_device (operands already in HBM) gives the host entry's c."""
     print("[gpu-case] device entry", name, flush=True)
-    n, width, limbs, q, a, b = workloads.baseline_case(name)
+    n, width, limbs, q, a, b = workloads.seeded_case(name)
     with polymul.Plan(n, width, limbs, q) as plan:
         expect = plan.multiply(a, b)
         da, db = device_memory.upload(a), device_memory.upload(b)
@@ -72,7 +72,7 @@ def test_split_entry_matches_multiply(polymul, workloads):
     """pm_multiply_begin / pm_multiply_finish give pm_multiply's c; a second
     begin before finish is refused; finish(NULL) discards and frees the plan."""
     print("[gpu-case] begin/finish", flush=True)
-    n, width, limbs, q, a, b = workloads.baseline_case("small")
+    n, width, limbs, q, a, b = workloads.seeded_case("small")
     with polymul.Plan(n, width, limbs, q) as plan:
         expect = plan.multiply(a, b)
         plan.multiply_begin(a, b)
