@@ -257,6 +257,91 @@ def cpu_baseline(q, a, b, plan_cls, sample_n):
     return cpu_s, scale, exact, reference
 
 
+PROFILE_ARGS = "--steps 1 --warmup 0 --no-cpu --no-secondary --no-batch"
+SQ_COUNTERS = ["SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_LDS_BANK_CONFLICT",
+               "SQ_LDS_IDX_ACTIVE", "SQ_WAIT_INST_LDS", "SQ_INSTS_LDS", "SQ_INSTS_VALU_MFMA_MOPS_I8"]
+
+
+def _short_kernel(name):
+    return name.replace("pm::(anonymous namespace)::", "").split("(")[0]
+
+
+def _counters(path):
+    """rocprofv3 counter_collection.csv -> {kernel: {counter: [value per dispatch]}}."""
+    import csv
+    from collections import defaultdict
+
+    values = defaultdict(lambda: defaultdict(list))
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            values[_short_kernel(row["Kernel_Name"])][row["Counter_Name"]].append(float(row["Counter_Value"]))
+    return values
+
+
+def summarize_traffic(fetch_csv, write_csv, dest):
+    """HBM bytes per dispatch from two counter runs of this script with
+    --calibrate-copy (FETCH_SIZE, then WRITE_SIZE; both in KiB). Each counter is
+    scaled by what it reads for the calibration copy, a plain 1 GiB
+    device-to-device copy (the largest copyBuffer dispatch): 1 GiB read, 1 GiB
+    written."""
+    copy_bytes = 1 << 30
+    fetch = {k: [x * 1024.0 for x in v["FETCH_SIZE"]] for k, v in _counters(fetch_csv).items() if "FETCH_SIZE" in v}
+    write = {k: [x * 1024.0 for x in v["WRITE_SIZE"]] for k, v in _counters(write_csv).items() if "WRITE_SIZE" in v}
+    copy = next(k for k in fetch if "copyBuffer" in k)
+    fetch_scale = copy_bytes / max(fetch[copy])
+    write_scale = copy_bytes / max(write[copy])
+    out = {"command": "rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) --output-format csv -- python3 bench.py "
+                      + PROFILE_ARGS + " --calibrate-copy",
+           "summary": "python3 bench.py --summarize-traffic <FETCH_SIZE counter_collection.csv> "
+                      "<WRITE_SIZE counter_collection.csv>",
+           "calibration": {"copy_bytes": copy_bytes, "fetch_scale": fetch_scale, "write_scale": write_scale},
+           "kernels": {}}
+    for name in sorted(set(fetch) & set(write)):
+        if "copyBuffer" in name:
+            continue
+        f = sum(fetch[name]) / len(fetch[name]) * fetch_scale
+        w = sum(write[name]) / len(write[name]) * write_scale
+        out["kernels"][name] = {"fetch_bytes": round(f), "write_bytes": round(w), "hbm_bytes": round(f + w)}
+    Path(dest).write_text(json.dumps(out, indent=1) + "\n")
+
+
+def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir):
+    """Markdown table of a --kernel-trace --stats run of the default product
+    (5 timed steps after 1 warm-up), plus the per-dispatch SQ counters of a
+    one-step counter run when pmc_csv is given."""
+    import csv
+
+    steps = "--steps 5 --warmup 1 --no-cpu --no-secondary --no-batch"
+    lines = ["# Kernel summary: plain product N=65536, L=1747 (55,904-bit random odd q)", "",
+             "Command (from the repo root, on one MI355X):", "", "```",
+             f"rocprofv3 --kernel-trace --stats --output-format csv -d {trace_dir} -o large -- python3 bench.py {steps}",
+             "```", "",
+             "6 products (1 warm-up + 5 timed). Source: `large_kernel_stats.csv`, summarised by "
+             "`python3 bench.py --summarize-kernels <stats csv> [<counter csv>]`.", "",
+             "| kernel | calls | avg ms | min ms | total ms | % |", "|---|---|---|---|---|---|"]
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            lines.append("| %s | %s | %.4f | %.4f | %.3f | %.2f |" % (
+                _short_kernel(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e6, float(r["MinNs"]) / 1e6,
+                float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+    if pmc_csv:
+        shown = ["SQ_VALU_MFMA_BUSY_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_LDS", "SQ_LDS_IDX_ACTIVE",
+                 "SQ_LDS_BANK_CONFLICT", "SQ_WAIT_INST_LDS"]
+        lines += ["", "## SQ counters (one product, per dispatch)", "", "```",
+                  f"rocprofv3 --pmc {' '.join(SQ_COUNTERS)} --output-format csv -d {pmc_dir} -o large -- "
+                  f"python3 bench.py {PROFILE_ARGS}",
+                  "```", "", "| kernel | " + " | ".join(shown) + " | conflict / LDS active |",
+                  "|---|" + "---|" * (len(shown) + 1)]
+        for k, d in _counters(pmc_csv).items():
+            if "copyBuffer" in k or "elementwise" in k:
+                continue
+            avg = {c: sum(v) / len(v) for c, v in d.items()}
+            act = avg.get("SQ_LDS_IDX_ACTIVE", 0)
+            ratio = "%.2f" % (avg.get("SQ_LDS_BANK_CONFLICT", 0) / act) if act else "-"
+            lines.append("| %s | " % k + " | ".join("%.4g" % avg.get(c, 0) for c in shown) + " | %s |" % ratio)
+    Path(dest).write_text("\n".join(lines) + "\n")
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -271,7 +356,22 @@ def main():
     parser.add_argument("--lanes", type=int, default=4)
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="also run one plain 1 GiB device-to-device copy (counter calibration)")
+    parser.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                        help="write profiles/traffic_large.json from two counter runs, then exit (no GPU)")
+    parser.add_argument("--summarize-kernels", nargs="+", metavar="CSV",
+                        help="write profiles/kernels_large.md from a kernel-stats csv and optionally an "
+                             "SQ counter csv, then exit (no GPU)")
     args = parser.parse_args()
+    if args.summarize_traffic:
+        summarize_traffic(*args.summarize_traffic, ROOT / "profiles" / "traffic_large.json")
+        return
+    if args.summarize_kernels:
+        stats = Path(args.summarize_kernels[0])
+        pmc = Path(args.summarize_kernels[1]) if len(args.summarize_kernels) > 1 else None
+        # the -d directories in the recorded commands are those the csv files came from
+        summarize_kernels(stats, pmc, ROOT / "profiles" / "kernels_large.md",
+                          str(stats.parent), str(pmc.parent) if pmc else None)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: one product runs on one GPU; the multi-GPU "
               "sharding (by primes, one reduce-scatter of CRT word sums) is a design, not a "
