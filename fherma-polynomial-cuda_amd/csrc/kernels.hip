@@ -646,16 +646,28 @@ __device__ __forceinline__ void row_inverse_group(uint32_t (&x)[8], int layout, 
     }
 }
 
-// Layouts C and D put up to 8 lanes of a wave on one LDS bank; an XOR swizzle
-// that removes those conflicts measured slower (its per-element addressing cost
-// more than the conflicts), so the exchange keeps plain indices, which the
-// compiler folds into immediate offsets.
+// LDS slot of jj in the exchange buffer: jj + jj/8, one pad word after every 8.
+// Unpadded, layouts C and D put 8 lanes of a wave on one bank (8-way conflicts);
+// padded, no layout exceeds 2-way.  Each form is written out as a per-thread base
+// plus a constant per element, so the padding costs no address arithmetic
+// (a swizzle computed per element measured slower than the conflicts).
+constexpr uint32_t kRowPadded = kRowC + kRowC / 8;
+
+__device__ __forceinline__ uint32_t padded_index(int layout, uint32_t t, int e) {
+    switch (layout) {
+        case 0: return t + (t >> 3) + 288u * e;
+        case 1: return 288u * (t >> 5) + (t & 31u) + ((t & 31u) >> 3) + 36u * e;
+        case 2: return 36u * (t >> 2) + (t & 3u) + 4u * e + (e >> 1);
+        default: return 9u * t + e;
+    }
+}
+
 __device__ __forceinline__ void row_exchange(uint32_t* buf, uint32_t (&x)[8], int from, int to, uint32_t t) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) buf[layout_index(from, t, e)] = x[e];
+    for (int e = 0; e < 8; ++e) buf[padded_index(from, t, e)] = x[e];
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = buf[layout_index(to, t, e)];
+    for (int e = 0; e < 8; ++e) x[e] = buf[padded_index(to, t, e)];
     __syncthreads();
 }
 
@@ -666,7 +678,7 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
                                                      const uint2* __restrict__ twf, const uint2* __restrict__ twi,
                                                      uint32_t ps, uint32_t n2, uint32_t rows, uint32_t r) {
     constexpr uint32_t n1 = N1;
-    __shared__ uint32_t buf[2][kRowC];
+    __shared__ uint32_t buf[2][kRowPadded];
     const uint32_t row = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
     const uint32_t p = consts[k].p, pinv = consts[k].pinv;
     uint32_t* pa = planes + size_t(k) * ps + size_t(row) * kRowC;
