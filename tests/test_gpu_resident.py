@@ -47,9 +47,10 @@ def test_operand_of_another_plan_is_rejected(polymul, workloads):
             assert err.value.status == 1
 
 
-@pytest.mark.parametrize("name", ["small", "mid"])
+@pytest.mark.parametrize("name", ["small", "mid", "large"])
 def test_device_entry_matches_host_entry(polymul, workloads, oracle, device_memory, name):
-    """pm_multiply_device (operands already in HBM) gives the host entry's c."""
+    """pm_multiply_device (operands already in HBM) gives the host entry's c,
+    up to the benchmark's own shape (N = 65536, L = 1747)."""
     print("[gpu-case] device entry", name, flush=True)
     n, width, limbs, q, a, b = workloads.seeded_case(name)
     with polymul.Plan(n, width, limbs, q) as plan:
