@@ -78,6 +78,7 @@ pm_status pm_multiply_finish(pm_plan* plan, uint32_t* c) {
 pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream) {
     return guarded([&] {
         require_device(plan);
+        require_idle(plan);
         if (!a || !b || !c) throw std::invalid_argument("null operand");
         hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->lane->stream;
@@ -92,6 +93,7 @@ pm_status pm_operand_create(pm_plan* plan, const uint32_t* b, pm_operand** out) 
         if (!out) throw std::invalid_argument("null output pointer");
         *out = nullptr;
         require_device(plan);
+        require_idle(plan);
         if (!b) throw std::invalid_argument("null operand");
         hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
         const pm::Shape& s = plan->host.shape;
@@ -138,6 +140,7 @@ pm_status pm_multiply_operand_device(pm_plan* plan, const uint32_t* a, const pm_
         require_device(plan);
         if (!a || !b || !c) throw std::invalid_argument("null operand");
         if (b->plan != plan) throw std::invalid_argument("operand belongs to another plan");
+        require_idle(plan);
         hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->lane->stream;
         plan->run(a, nullptr, c, PM_STAGE_OUTPUT, st, b->planes);
@@ -227,6 +230,7 @@ pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, in
         }
         if (*bytes < need) throw std::invalid_argument("stage buffer too small");
         require_device(plan);
+        require_idle(plan);
         if (!a || !b) throw std::invalid_argument("null operand");
         if (s.trivial && stage != PM_STAGE_OUTPUT) throw std::invalid_argument("q == 1 has no intermediate stages");
         hip_check(hipSetDevice(plan->device_id), "hipSetDevice");

@@ -7,9 +7,10 @@
 // their staging from here and give it back when they are destroyed or grow:
 // the next plan of a similar size reuses populated memory.  A request takes
 // the smallest cached block that holds it, if that block is at most twice its
-// size; otherwise a new block is allocated.  Cached bytes are capped (a
-// quarter of physical memory, at most 32 GiB); beyond that the largest cached
-// blocks are freed.  Blocks still cached at exit are left to the OS.
+// size; otherwise a new block is allocated (if pinning fails, every cached
+// block is freed and the allocation tried once more).  Cached bytes are capped
+// (a quarter of physical memory, at most 32 GiB); beyond that the largest
+// cached blocks are freed.  Blocks still cached at exit are left to the OS.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,8 +45,12 @@ public:
                 return p;
             }
         }
-        void* p = nullptr;
-        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) throw std::bad_alloc();
+        void* p = allocate(bytes);
+        if (!p) {  // pinned memory exhausted: give back what the cache holds and try once more
+            release_all();
+            p = allocate(bytes);
+            if (!p) throw std::bad_alloc();
+        }
         *got = bytes;
         return p;
     }
@@ -64,6 +69,20 @@ public:
     }
 
 private:
+    static void* allocate(size_t bytes) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess && p) return p;
+        (void)hipGetLastError();  // clear the failure so later calls do not report it
+        return nullptr;
+    }
+
+    void release_all() {
+        std::lock_guard<std::mutex> lock(mutex_);
+        for (auto& [size, block] : blocks_) (void)hipHostFree(block);
+        blocks_.clear();
+        cached_ = 0;
+    }
+
     PinnedCache() {
         const long pages = sysconf(_SC_PHYS_PAGES), page = sysconf(_SC_PAGE_SIZE);
         const size_t quarter = pages > 0 && page > 0 ? size_t(pages) * size_t(page) / 4 : size_t(4) << 30;

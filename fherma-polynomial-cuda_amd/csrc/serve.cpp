@@ -41,6 +41,7 @@ struct Job {
     std::unique_ptr<pm::HostPlan> host;
     std::exception_ptr plan_error;
     bool planned = false, taken = false;
+    bool claimed = false;  // a pm_server_wait holds this ticket (guarded by the server's mutex)
 };
 
 constexpr unsigned kPlanners = 2;
@@ -239,6 +240,11 @@ pm_status pm_server_wait(pm_server* server, uint64_t ticket, pm_request_times* t
             return PM_ERR_INVALID_ARGUMENT;
         }
         Job* j = it->second.get();
+        if (j->claimed) {  // the first waiter frees the job when it returns
+            g_last_error = "ticket is already being waited for";
+            return PM_ERR_INVALID_ARGUMENT;
+        }
+        j->claimed = true;
         server->done.wait(lock, [&] { return j->finished; });
         job = std::move(it->second);
         server->jobs.erase(it);

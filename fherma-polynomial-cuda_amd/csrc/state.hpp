@@ -557,3 +557,9 @@ inline void require_device(const pm_plan* plan) {
     if (!plan->device) throw NoDevice("plan has no device state (created with pm_plan_create_host)");
 }
 
+// Entry points that use the plan's device buffers wait for a product begun
+// with pm_multiply_begin to be finished first: its kernels still use them.
+inline void require_idle(const pm_plan* plan) {
+    if (plan->pending) throw std::invalid_argument("the plan's begun product has not been finished");
+}
+

@@ -6,6 +6,7 @@ device pointers.  It never computes a product itself: if the shared library is
 missing, loading fails with an error telling how to build it.
 """
 import ctypes
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -340,7 +341,9 @@ class Server:
         handle = ctypes.c_void_p()
         _check(self.lib, self.lib.pm_server_create(int(device_budget_bytes), int(lanes), ctypes.byref(handle)))
         self.handle = handle
-        self._pending = {}
+        self._pending = {}  # ticket -> the request and the buffers it keeps alive
+        self._waiting = set()
+        self._lock = threading.Lock()
 
     def submit(self, n, limbs, q, a, b):
         req, c, keep = _request(n, limbs, q, a, b)
@@ -350,9 +353,20 @@ class Server:
         return ticket.value
 
     def wait(self, ticket):
+        # A ticket is waited once; its buffers stay referenced until that wait
+        # has returned, so a second waiter must not release them.
+        with self._lock:
+            if ticket not in self._pending or ticket in self._waiting:
+                raise PolymulError(PM_ERR_INVALID_ARGUMENT, "unknown ticket or one already being waited for")
+            self._waiting.add(ticket)
         times = RequestTimes()
-        status = self.lib.pm_server_wait(self.handle, ticket, ctypes.byref(times))
-        _, c, _ = self._pending.pop(ticket)
+        try:
+            status = self.lib.pm_server_wait(self.handle, ticket, ctypes.byref(times))
+        finally:
+            with self._lock:
+                self._waiting.discard(ticket)
+        with self._lock:
+            _, c, _ = self._pending.pop(ticket)
         _check(self.lib, status)
         return c, {"queue_ms": times.queue_ms, "service_ms": times.service_ms, "total_ms": times.total_ms}
 

@@ -75,8 +75,9 @@ pm_status pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint3
  * vector; the reference builds it concurrently too, output_worker.h:17-75).
  * pm_multiply_begin copies a and b in and queues every device step; it returns
  * with the device busy.  pm_multiply_finish waits and writes c (c == NULL: waits
- * and discards the result).  a and b may be released after begin returns; no
- * other product may use the plan in between. */
+ * and discards the result).  a and b may be released after begin returns.  In
+ * between, every other product, operand or stage call on the plan fails with
+ * PM_ERR_INVALID_ARGUMENT (the begun product still uses its buffers). */
 pm_status pm_multiply_begin(pm_plan* plan, const uint32_t* a, const uint32_t* b);
 pm_status pm_multiply_finish(pm_plan* plan, uint32_t* c);
 
@@ -174,7 +175,9 @@ pm_status pm_server_create(uint64_t device_budget_bytes, uint32_t lanes, pm_serv
 /* Queues a request (first in, first out); returns at once with its ticket. */
 pm_status pm_server_submit(pm_server* server, const pm_request* request, uint64_t* ticket);
 /* Blocks until the request has finished; returns the request's own status (its
- * message through pm_last_error) and its times.  Each ticket is waited once. */
+ * message through pm_last_error) and its times.  Each ticket is waited once:
+ * a second wait, also one from another thread while the first still blocks,
+ * fails with PM_ERR_INVALID_ARGUMENT. */
 pm_status pm_server_wait(pm_server* server, uint64_t ticket, pm_request_times* times);
 /* Completes every queued request, then releases the lanes and the arena. */
 void pm_server_destroy(pm_server* server);

@@ -84,3 +84,28 @@ def test_split_entry_matches_multiply(polymul, workloads):
         plan.multiply_begin(b, a)
         polymul.load_library().pm_multiply_finish(plan.handle, None)  # discard
         assert np.array_equal(plan.multiply(a, b), expect)
+
+
+def test_begun_product_blocks_the_plans_other_entries(polymul, workloads, device_memory):
+    """Between begin and finish the device entry, operand creation and the
+    stage tap are refused (they would reuse the buffers the begun product's
+    queued kernels still use); the begun product's result is unaffected."""
+    print("[gpu-case] begun product blocks other entries", flush=True)
+    n, width, limbs, q, a, b = workloads.seeded_case("small")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        expect = plan.multiply(a, b)
+        da, db = device_memory.upload(b), device_memory.upload(a)
+        dc = device_memory.alloc(a.nbytes)
+        try:
+            plan.multiply_begin(a, b)
+            for call in (lambda: plan.multiply_device(da, db, dc), lambda: plan.operand(b),
+                         lambda: plan.debug_stage(a, b, polymul.STAGE_OUTPUT)):
+                with pytest.raises(polymul.PolymulError) as err:
+                    call()
+                assert err.value.status == polymul.PM_ERR_INVALID_ARGUMENT
+            assert np.array_equal(plan.multiply_finish(), expect)
+            plan.multiply_device(da, db, dc)  # accepted again once finished
+            assert np.array_equal(device_memory.download(dc, a.shape, np.uint32), plan.multiply(b, a))
+        finally:
+            for ptr in (da, db, dc):
+                device_memory.free(ptr)

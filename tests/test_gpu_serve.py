@@ -93,3 +93,29 @@ def test_invalid_request_fails_alone(polymul, workloads):
         assert err.value.status == polymul.PM_ERR_INVALID_ARGUMENT
         c, _ = server.wait(good)
     assert np.array_equal(c, _alone(polymul, n, limbs, q, a, b))
+
+
+def test_a_ticket_is_waited_once(polymul, workloads):
+    """Two threads waiting on one ticket: one gets the result, the other
+    PM_ERR_INVALID_ARGUMENT (whichever order they arrive in), and a later wait
+    on the same ticket is refused as well."""
+    import threading
+
+    print("[gpu-case] server double wait", flush=True)
+    n, limbs, q, a, b = _product(workloads, 65536, 55, 9500)
+    lib = polymul.load_library()
+    with polymul.Server(1 * GiB, lanes=1) as server:
+        ticket = server.submit(n, limbs, q, a, b)
+        statuses = []
+        threads = [threading.Thread(target=lambda: statuses.append(lib.pm_server_wait(server.handle, ticket, None)))
+                   for _ in range(2)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        assert sorted(statuses) == [polymul.PM_OK, polymul.PM_ERR_INVALID_ARGUMENT]
+        _, c, _ = server._pending.pop(ticket)  # waited through the C ABI above
+        with pytest.raises(polymul.PolymulError) as err:
+            server.wait(ticket)
+        assert err.value.status == polymul.PM_ERR_INVALID_ARGUMENT
+    assert np.array_equal(c, _alone(polymul, n, limbs, q, a, b))
