@@ -556,7 +556,14 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t*
                                                                  uint32_t ksteps, int8_t* __restrict__ tiles) {
     __shared__ uint32_t lds[ROWS][16][17];
     const uint32_t kk = threadIdx.x >> 4, cc = threadIdx.x & 15;
-    const uint32_t c0 = blockIdx.x * 16, kg = blockIdx.y, k = kg * 16 + kk;
+    // A block reads 64 bytes of each row: half a 128-byte line, whose other half
+    // is the next column group's.  Workgroups go to the 8 XCDs round robin by
+    // linear id, so column groups 2m and 2m+1 are given to blocks 8 ids apart
+    // (the same XCD when the grid's x extent is a multiple of 16): the second
+    // read of each line hits that XCD's L2 instead of memory.
+    const uint32_t bx = blockIdx.x;
+    const uint32_t cx = (gridDim.x & 15u) == 0 ? (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u) : bx;
+    const uint32_t c0 = cx * 16, kg = blockIdx.y, k = kg * 16 + kk;
     uint32_t x[ROWS];
     if (k < r) {
         const uint32_t p = consts[k].p;
