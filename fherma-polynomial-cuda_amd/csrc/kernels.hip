@@ -264,13 +264,27 @@ __global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict_
     const uint32_t op = op0 + gr / per_op, rr = gr % per_op;
     const uint32_t tn = tile0 + rr / kTile, row = rr % kTile, coef = tn * kTile + row;
     const bool live = coef < n;
+    // Rows past n read (and discard) row 0 of their operand, so every load is
+    // unconditional: kPackLoads loads per lane are in flight before their
+    // stores, instead of one load, a wait and a store at a time.
     const uint32_t* src = (op ? b : a) + size_t(live ? coef : 0) * limbs;
     uint32_t* dst = packed + (size_t(op) * ntiles + tn) * ksteps * (kTileBytes / 4) + row * (kTK / 4);
     const uint32_t words = ksteps * (kTK / 4);
-#pragma unroll 4
-    for (uint32_t w = lane; w < words; w += 64) {
-        const uint32_t v = (live && w < limbs) ? src[w] : 0u;
-        dst[size_t(w / (kTK / 4)) * (kTileBytes / 4) + w % (kTK / 4)] = v ^ 0x80808080u;
+    constexpr uint32_t kPackLoads = 8;
+    for (uint32_t w0 = lane; w0 < words; w0 += 64 * kPackLoads) {
+        uint32_t v[kPackLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kPackLoads; ++u) {
+            const uint32_t w = w0 + 64 * u;
+            v[u] = src[w < limbs ? w : limbs - 1];  // clamped into the row; padding words are zeroed below
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kPackLoads; ++u) {
+            const uint32_t w = w0 + 64 * u;
+            if (w < words)
+                dst[size_t(w / (kTK / 4)) * (kTileBytes / 4) + w % (kTK / 4)] =
+                    ((live && w < limbs) ? v[u] : 0u) ^ 0x80808080u;
+        }
     }
 }
 
