@@ -459,6 +459,19 @@ def main():
         result["reference_point"] = {"N": 65536, "W_bits": 1747, "L": 55, "ms_device": round(rms, 4),
                                      "fherma_run_ms": harness,
                                      "published_ms_e2e_rtx_pro_6000": 1.0712435}
+        # the smaller configurations (N=1024 with a 256-bit q, N=32768 with a
+        # 27,776-bit q): kernel-only and transfer-inclusive ms per product
+        others = {}
+        for name, label in (("small", "B-small"), ("mid", "B-mid")):
+            sn, sw, sl, sq, sa, sb = workloads.seeded_case(name)
+            with polymul.Plan(sn, sw, sl, sq) as splan:
+                dms, _, dc = time_device(splan, torch, sa, sb, max(args.steps, 10), args.warmup)
+                splan.set_profiling(False)
+                host_ms, hc = time_host(splan, sa, sb, 5)
+            others[label] = {"n": sn, "limbs": sl, "ms_device": round(dms, 4),
+                             "e2e_host_ms": round(float(np.median(host_ms)), 3),
+                             "e2e_matches_device": bool(np.array_equal(hc, dc))}
+        result["other_configs"] = others
 
     if not args.no_batch:
         # one untimed small product first: HIP and kernel-module start-up belong to neither leg
