@@ -133,6 +133,21 @@ def test_stress_boundaries(polymul, oracle, n, width):
 
 
 # ---------------------------------------------------------------------------
+# the largest transforms: N = 2^20 (64-bit q) and N = 2^24, the largest N a plan
+# accepts (32-bit q); random odd q, full comparison with the oracle
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,limbs", [(1 << 20, 2), (1 << 24, 1)])
+def test_largest_transforms(polymul, workloads, oracle, n, limbs):
+    _announce("largest-transforms", n, limbs)
+    rng = np.random.default_rng(7000 + limbs)
+    q = workloads.random_modulus(limbs, rng)
+    a = workloads.uniform_below(q, n, rng)
+    b = workloads.uniform_below(q, n, rng)
+    with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+        _check_product(plan, a, b, q, oracle)
+
+
+# ---------------------------------------------------------------------------
 # the plain configurations with random odd q
 # ---------------------------------------------------------------------------
 def _edge_operands(workloads, q_words, n, kind):
