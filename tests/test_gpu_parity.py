@@ -147,6 +147,23 @@ def test_largest_transforms(polymul, workloads, oracle, n, limbs):
         _check_product(plan, a, b, q, oracle)
 
 
+# the widest q: 131,072 bits (L = 4096, about 8,460 primes); N = 1 is a plain
+# modular product of two such integers
+@pytest.mark.parametrize("n", [1, 64])
+def test_widest_moduli(polymul, workloads, oracle, n):
+    limbs = 4096
+    _announce("widest-moduli", n, limbs)
+    rng = np.random.default_rng(8000 + n)
+    q = workloads.random_modulus(limbs, rng)
+    a = workloads.uniform_below(q, n, rng)
+    b = workloads.uniform_below(q, n, rng)
+    with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+        assert plan.info()["primes"] > 8000
+        _check_product(plan, a, b, q, oracle)
+        m = np.tile(workloads.q_minus_one(q), (n, 1))
+        _check_product(plan, m, m, q, oracle)
+
+
 # ---------------------------------------------------------------------------
 # the plain configurations with random odd q
 # ---------------------------------------------------------------------------
