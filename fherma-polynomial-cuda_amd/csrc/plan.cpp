@@ -397,14 +397,14 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
             break;
         }
     }
-    if (!found) throw std::runtime_error("insufficient independent NTT fields");
+    if (!found) throw InsufficientFields("no prime family of any transform split covers 4N(q-1)^2");
     s.n1 = n1;
     s.n2 = n / n1;
     s.log_n2 = 0;
     while ((1u << s.log_n2) < s.n2) ++s.log_n2;
     s.reference_walk = (n1 == 1);
     s.r = uint32_t(plan.primes.size());
-    if (s.r >= 16384) throw std::runtime_error("modulus too wide: more than 16383 NTT fields needed");
+    if (s.r >= 16384) throw InsufficientFields("the modulus needs more than 16383 primes");
     s.n_pad = (n + 255) / 256 * 256;
     s.r_pad = (s.r + 63) / 64 * 64;
     s.conv_k = (4 * limbs + 63) / 64 * 64;
@@ -417,7 +417,7 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
     s.row_width = n < 2048 ? n : 2048;
     if (s.row_width < n1) s.row_width = n1;
     s.rows = n / s.row_width;
-    if (s.row_width > 2048) throw std::runtime_error("insufficient independent NTT fields");
+    if (s.row_width > 2048) throw InsufficientFields("the transform split leaves blocks wider than 2048");
     if (s.rows > 8192) throw std::invalid_argument("invalid point: transform exceeds the supported size (N <= 2^24)");
 
     const uint32_t r = s.r, L = limbs, n2 = s.n2;

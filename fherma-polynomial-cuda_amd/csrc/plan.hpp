@@ -4,12 +4,20 @@
 #pragma once
 
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "bigint.hpp"
 
 namespace pm {
+
+// No set of NTT primes the plan can use covers 4N(q-1)^2 (the reference's
+// "insufficient independent NTT fields", solve.cu:910); PM_ERR_INSUFFICIENT_FIELDS.
+struct InsufficientFields : std::runtime_error {
+    explicit InsufficientFields(const std::string& what)
+        : std::runtime_error("insufficient independent NTT fields: " + what) {}
+};
 
 // Per-prime constants read by the kernels.
 struct PrimeConst {

@@ -541,11 +541,9 @@ pm_status guarded(F&& body) {
     } catch (const std::bad_alloc&) {
         g_last_error = "out of memory";
         return PM_ERR_OUT_OF_MEMORY;
-    } catch (const std::runtime_error& e) {
+    } catch (const pm::InsufficientFields& e) {
         g_last_error = e.what();
-        return std::string(e.what()).find("insufficient independent NTT fields") != std::string::npos
-                   ? PM_ERR_INSUFFICIENT_FIELDS
-                   : PM_ERR_INTERNAL;
+        return PM_ERR_INSUFFICIENT_FIELDS;
     } catch (const std::exception& e) {
         g_last_error = e.what();
         return PM_ERR_INTERNAL;

@@ -26,7 +26,8 @@ extern "C" {
 typedef enum pm_status {
     PM_OK = 0,
     PM_ERR_INVALID_ARGUMENT = 1,  /* bad point or input size (solve.cu:903-904, 1228) */
-    PM_ERR_INSUFFICIENT_FIELDS = 2, /* no prime family covers 4N(q-1)^2 (solve.cu:910) */
+    PM_ERR_INSUFFICIENT_FIELDS = 2, /* no usable prime family covers 4N(q-1)^2 (solve.cu:910),
+                                       e.g. q needs more than 16383 primes */
     PM_ERR_DEVICE = 3,            /* a HIP call failed (solve.cu:80-82) */
     PM_ERR_OUT_OF_MEMORY = 4,
     PM_ERR_INTERNAL = 5,

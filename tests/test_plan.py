@@ -272,6 +272,12 @@ def test_invalid_points_and_host_only_plans(polymul, oracle):
     with pytest.raises(polymul.PolymulError) as err:
         polymul.Plan(1 << 28, 64, 2, oracle.int_to_words((1 << 61) - 1, 2), device=False)
     assert err.value.status == 1  # transform beyond the supported size
+    # a 262,144-bit q needs about 16,900 primes, past the 16,383 a plan supports
+    wide = oracle.int_to_words((1 << (32 * 8192 - 1)) + 12345, 8192)
+    with pytest.raises(polymul.PolymulError) as err:
+        polymul.Plan(1024, 32 * 8192, 8192, wide, device=False)
+    assert err.value.status == polymul.PM_ERR_INSUFFICIENT_FIELDS
+    assert "insufficient independent NTT fields" in str(err.value)
     with polymul.Plan(8, 1, 1, oracle.int_to_words(1, 1), device=False) as plan:
         assert plan.info()["trivial"] == 1  # q == 1: the zero ring
     with polymul.Plan(8, 16, 1, oracle.int_to_words(65521, 1), device=False) as plan:
