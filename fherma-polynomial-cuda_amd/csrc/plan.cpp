@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <exception>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -89,16 +90,33 @@ unsigned plan_threads() {
 
 // fn(begin, end) over [0, count) split across up to plan_threads() host
 // threads (the calling thread takes the first share); serial below
-// `min_share` per thread.  fn must not throw.
+// `min_share` per thread.  Every thread is joined before the first exception a
+// share threw (std::bad_alloc, say) is rethrown; shares whose thread could not
+// be started run on the calling thread.
 template <class F>
 void parallel_range(size_t count, size_t min_share, F&& fn) {
     unsigned threads = plan_threads();
     threads = unsigned(std::max<size_t>(1, std::min<size_t>(threads, count / std::max<size_t>(1, min_share))));
+    std::vector<std::exception_ptr> errors(threads);
+    auto share = [&fn, &errors, count, threads](unsigned t) {
+        try {
+            fn(count * t / threads, count * (t + 1) / threads);
+        } catch (...) {
+            errors[t] = std::current_exception();
+        }
+    };
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < threads; ++t)
-        pool.emplace_back([&fn, count, threads, t] { fn(count * t / threads, count * (t + 1) / threads); });
-    fn(0, count / threads);
+    pool.reserve(threads);
+    unsigned started = 1;
+    try {
+        for (; started < threads; ++started) pool.emplace_back(share, started);
+    } catch (...) {  // no more threads available
+    }
+    for (unsigned t = started; t < threads; ++t) share(t);
+    share(0);
     for (auto& th : pool) th.join();
+    for (const auto& e : errors)
+        if (e) std::rethrow_exception(e);
 }
 
 // a mod p_k for each of `count` primes p_k < 2^31, with rcp_k =
