@@ -82,6 +82,13 @@ __device__ __forceinline__ uint32_t redc(uint64_t t, uint32_t p, uint32_t pinv) 
     return hi < mh ? r + p : r;
 }
 
+// REDC for t < 2p*2^32 (four products of residues: 4p^2 < 2p*2^32 as 2p < 2^32):
+// hi(t) - hi(m*p) lies in (-p, 2p); one more conditional subtraction than redc.
+__device__ __forceinline__ uint32_t redc_wide(uint64_t t, uint32_t p, uint32_t pinv) {
+    const uint32_t r = redc(t, p, pinv);  // in [0, 2p) for such t
+    return r >= p ? r - p : r;
+}
+
 __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b, uint32_t p, uint32_t pinv) {
     return redc(uint64_t(a) * b, p, pinv);
 }
@@ -730,52 +737,50 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
     row_forward_group(xa, 3, 2, t, 1, n1, base_index, wf, p);
     row_forward_group(xb, 3, 2, t, 1, n1, base_index, wf, p);
 
-    // base case: thread t holds jj = 8t .. 8t+7, i.e. 8/n1 whole blocks
+    // base case: thread t holds jj = 8t .. 8t+7, i.e. 8/n1 whole blocks.  Block g
+    // is reduced mod (X^n1 - zeta_g): c_s = sum_{u<=s} a_u b_{s-u} + sum_{u>s}
+    // a_u (zeta_g b_{s+n1-u}).  The wrapped factors zeta_g*b_v are formed once per
+    // block (Shoup), so each output is one sum of n1 products, reduced four at a
+    // time (a sum of four products of residues is below 4p^2 < 2p*2^32, in
+    // redc_wide's range; 32-bit multiplies are the scarce resource here).
     uint32_t xc[8];
+    constexpr int n1i = int(N1);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        constexpr int n1i = int(N1);
-        const int s1 = s % n1i, blk0 = s - s1;  // block start inside the 8 values
-        // Products are summed two at a time before one REDC: a*b + c*d < 2p^2 <
-        // p*2^32 (p < 2^31), inside REDC's range, so a pair costs one reduction
-        // (two multiplies) less and lands on the same residue.  (32-bit
-        // multiplies are quarter rate and bound this kernel.)
-        uint32_t lo = 0, hi = 0;
-        uint64_t acc_lo = 0, acc_hi = 0;
-        int pending_lo = 0, pending_hi = 0;
-#pragma unroll
-        for (int u = 0; u < n1i; ++u) {
-            const uint64_t prod = uint64_t(xa[blk0 + u]) * xb[blk0 + (s1 >= u ? s1 - u : s1 + n1i - u)];
-            if (u <= s1) {
-                acc_lo += prod;
-                if (++pending_lo == 2) {
-                    lo = add_mod(lo, redc(acc_lo, p, pinv), p);
-                    acc_lo = 0;
-                    pending_lo = 0;
-                }
-            } else {
-                acc_hi += prod;
-                if (++pending_hi == 2) {
-                    hi = add_mod(hi, redc(acc_hi, p, pinv), p);
-                    acc_hi = 0;
-                    pending_hi = 0;
-                }
-            }
-        }
-        if (pending_lo) lo = add_mod(lo, redc(acc_lo, p, pinv), p);
-        if (pending_hi) hi = add_mod(hi, redc(acc_hi, p, pinv), p);
-        uint32_t value = lo;
-        if (n1 > 1) {
+    for (int blk0 = 0; blk0 < 8; blk0 += n1i) {
+        uint32_t bw[n1i];  // bw[v] = zeta_g * b_v for v >= 1
+        bw[0] = 0;
+        if constexpr (n1i > 1) {
             const uint32_t gb = (row * kRowC + 8 * t + blk0) / n1;  // global block index
-            if (n2 == 1) {
-                value = sub_mod(lo, hi, p);
-            } else {
-                const uint2 w = wf[n2 / 2 + gb / 2];
-                const uint32_t z = mul_shoup(hi, w.x, w.y, p);
-                value = (gb & 1) ? sub_mod(lo, z, p) : add_mod(lo, z, p);
+            uint2 w{0, 0};
+            if (n2 != 1) w = wf[n2 / 2 + gb / 2];  // zeta_{2g} = w, zeta_{2g+1} = -w
+#pragma unroll
+            for (int v = 1; v < n1i; ++v) {
+                const uint32_t b = xb[blk0 + v];
+                uint32_t z;
+                if (n2 == 1) {
+                    z = b ? p - b : 0;  // zeta = -1
+                } else {
+                    z = mul_shoup(b, w.x, w.y, p);
+                    if (gb & 1) z = z ? p - z : 0;
+                }
+                bw[v] = z;
             }
         }
-        xc[s] = value;
+#pragma unroll
+        for (int s1 = 0; s1 < n1i; ++s1) {
+            uint64_t acc = 0;
+            uint32_t value = 0;
+#pragma unroll
+            for (int u = 0; u < n1i; ++u) {
+                const uint32_t f = u <= s1 ? xb[blk0 + s1 - u] : bw[s1 + n1i - u];
+                acc += uint64_t(xa[blk0 + u]) * f;
+                if ((u & 3) == 3 || u == n1i - 1) {
+                    value = add_mod(value, redc_wide(acc, p, pinv), p);
+                    acc = 0;
+                }
+            }
+            xc[blk0 + s1] = value;
+        }
     }
 
     row_inverse_group(xc, 3, 2, t, 1, n1, base_index, wi, p);
