@@ -835,7 +835,7 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     const uint32_t sw = args.s_shift / 32, sb = args.s_shift % 32;
     uint32_t top[3] = {0, 0, 0};
     {
-        constexpr uint32_t kG1 = 16;
+        constexpr uint32_t kG1 = 32;
         int64_t da[kG1], db[kG1];
         int64_t carry = 0;
         auto fetch = [&](int64_t (&d)[kG1], uint32_t j0) {
