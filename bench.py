@@ -89,23 +89,6 @@ def time_host(plan, a, b, runs):
 GiB = 1 << 30
 
 
-def log_uniform_int(rng, lo, hi):
-    return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
-
-
-def make_products(workloads, count, seed, n_choices, limb_range):
-    """Independent products, each with its own seeded random odd q and uniform operands."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        n = int(n_choices[rng.integers(0, len(n_choices))])
-        limbs = log_uniform_int(rng, *limb_range)
-        prng = np.random.default_rng([seed, i])
-        q = workloads.random_modulus(limbs, prng)
-        out.append((n, limbs, q, workloads.uniform_below(q, n, prng), workloads.uniform_below(q, n, prng)))
-    return out
-
-
 def plain_products(polymul, products):
     """One at a time, as a harness would run fherma_init + fherma_run per point:
     a plan for the product's own q, host vectors in and out."""
@@ -125,7 +108,7 @@ def batched_leg(polymul, workloads, count, lanes, passes=3):
     time, first launches at each shape: both forms pay these once per process),
     then `passes` timed passes over the same 64 products; products/s is from
     the median pass.  The untimed first passes are reported as *_cold."""
-    products = make_products(workloads, count, 0x5EED0004, [4096, 8192, 16384, 32768, 65536], (16, 1747))
+    products = workloads.batch_products(count)
 
     def timed(fn):
         t0 = time.perf_counter()
@@ -148,12 +131,9 @@ def batched_leg(polymul, workloads, count, lanes, passes=3):
             "host_link_gb": round(12 * words / 1e9, 2)}, products
 
 
-def resident_leg(polymul, workloads, count, n, limbs):
+def resident_leg(polymul, workloads, count):
     """64 random a times one fixed b: plain products vs b's residues kept on the device."""
-    rng = np.random.default_rng(0x5EED0005)
-    q = workloads.random_modulus(limbs, rng)
-    b = workloads.uniform_below(q, n, rng)
-    a_list = [workloads.uniform_below(q, n, rng) for _ in range(count)]
+    n, limbs, q, b, a_list = workloads.resident_case(count)
     with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
         plan.multiply(a_list[0], b)
         t0 = time.perf_counter()
@@ -174,7 +154,7 @@ def served_leg(polymul, workloads, count, lanes, rates):
     log-uniform in [32, 1747] limbs, own q each; 200 GB admission budget.
     Latency = submission to result (server clock); every result must equal the
     same request run alone through the plain path."""
-    products = make_products(workloads, count, 0x5EED0006, [4096, 8192, 16384, 32768, 65536], (32, 1747))
+    products = workloads.served_products(count)
     alone, alone_ms = plain_products(polymul, products)
     sweeps = []
     for rate in rates:
@@ -255,6 +235,65 @@ def cpu_baseline(q, a, b, plan_cls, sample_n):
     bits_full = n_full * ((n_full * (reference.words_to_int(q) - 1) ** 2).bit_length())
     scale = (bits_full * math.log2(bits_full)) / (bits_sample * math.log2(bits_sample))
     return cpu_s, scale, exact, reference
+
+
+def calibrate():
+    """Register-only throughput of the instructions the rooflines are built on
+    (libpm_calibrate.so): INT8 MFMA 16x16x64 and 32x32x32 with every operand in
+    registers, and the 32-bit multiply rate (v_mul_lo_u32 + v_mul_hi_u32) that
+    bounds the NTT row stage.  256 CUs x 8 blocks, 8 waves per block."""
+    import ctypes
+
+    lib = ctypes.CDLL(str(PKG / "libpm_calibrate.so"))
+    ms, ops = ctypes.c_float(), ctypes.c_double()
+    out = {}
+    for shape in (16, 32):
+        rc = lib.pm_calibrate_mfma_i8(shape, 256 * 8, 8, 2048, ctypes.byref(ms), ctypes.byref(ops))
+        if rc == 0:
+            out["mfma_i8_%dx%d_tops" % (shape, shape)] = round(ops.value / (ms.value * 1e-3) / 1e12, 1)
+    if lib.pm_calibrate_mul32(256 * 16, 4096, ctypes.byref(ms), ctypes.byref(ops)) == 0:
+        out["mul32_per_s"] = float("%.4g" % (ops.value / (ms.value * 1e-3)))
+    return out
+
+
+def _oracle_worker(name):
+    """Child process: one GMP Kronecker product (oracle, 1 core) of the seeded
+    workload `name`; prints its monotonic start and end times as JSON."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import reference
+
+    workloads = _load("workloads")
+    n, width, limbs, q, a, b = workloads.seeded_case(name)
+    t0 = time.monotonic()
+    reference.kronecker_gmp(a, b, q)
+    print(json.dumps({"start": t0, "end": time.monotonic()}), flush=True)
+
+
+def _oracle_processes(name, count):
+    """`count` concurrent oracle products of workload `name`, one process (and
+    core) each; returns (seconds of each, wall seconds from first start to last end)."""
+    import subprocess
+
+    procs = [subprocess.Popen([sys.executable, str(ROOT / "bench.py"), "--oracle-worker", name],
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for _ in range(count)]
+    spans = []
+    for proc in procs:
+        out, err = proc.communicate(timeout=900)
+        if proc.returncode != 0:
+            raise RuntimeError("oracle worker failed: %s" % err[-500:])
+        spans.append(json.loads(out.strip().splitlines()[-1]))
+    wall = max(x["end"] for x in spans) - min(x["start"] for x in spans)
+    return [x["end"] - x["start"] for x in spans], wall
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 PROFILE_ARGS = "--steps 1 --warmup 0 --no-cpu --no-secondary --no-batch"
@@ -354,6 +393,8 @@ def main():
     parser.add_argument("--batch-count", type=int, default=64)
     parser.add_argument("--serve-count", type=int, default=48)
     parser.add_argument("--lanes", type=int, default=4)
+    parser.add_argument("--oracle-worker", metavar="CONFIG", help=argparse.SUPPRESS)
+    parser.add_argument("--no-calibrate", action="store_true", help="skip the register-only peak kernels")
     parser.add_argument("--calibrate-copy", action="store_true",
                         help="also run one plain 1 GiB device-to-device copy (counter calibration)")
     parser.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -362,6 +403,9 @@ def main():
                         help="write profiles/kernels_large.md from a kernel-stats csv and optionally an "
                              "SQ counter csv, then exit (no GPU)")
     args = parser.parse_args()
+    if args.oracle_worker:
+        _oracle_worker(args.oracle_worker)
+        return
     if args.summarize_traffic:
         summarize_traffic(*args.summarize_traffic, ROOT / "profiles" / "traffic_large.json")
         return
@@ -432,6 +476,13 @@ def main():
     result["roofline"] = {"kernel": "conversion_gemm", "bound": "mfma", "achieved": round(achieved, 1),
                           "peak": INT8_PEAK_TOPS, "unit": "TOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
                           "traffic": traffic, "ops_per_launch": ops, "avg_launch_ms": round(conv_ms, 4)}
+    if not args.no_calibrate:
+        peaks = calibrate()
+        result["measured_peaks"] = peaks
+        measured = peaks.get("mfma_i8_16x16_tops")
+        if measured:
+            result["roofline"]["measured_peak"] = measured
+            result["roofline"]["frac_of_measured_peak"] = round(achieved / measured, 4)
     result["stages_ms"] = {k: round(v, 4) for k, v in stages.items()}
     result["crt_gemm_tops"] = round(crt_ops(n, limbs, r) / (stages["crt_gemm"] * 1e-3) / 1e12, 1)
     result["setup_s"] = {"inputs": round(gen_s, 2), "plan": round(plan_s, 2)}
@@ -475,11 +526,11 @@ def main():
 
     if not args.no_batch:
         # one untimed small product first: HIP and kernel-module start-up belong to neither leg
-        warm = make_products(workloads, 1, 1, [4096], (16, 17))
+        warm = workloads.make_products(1, 1, [4096], (16, 17))
         plain_products(polymul, warm)
         batched, _ = batched_leg(polymul, workloads, args.batch_count, args.lanes)
         result["batched"] = batched
-        result["resident_operand"] = resident_leg(polymul, workloads, 64, 65536, 55)
+        result["resident_operand"] = resident_leg(polymul, workloads, 64)
         capacity = batched["batched_products_per_s"]
         result["served"] = served_leg(polymul, workloads, args.serve_count, args.lanes,
                                       [0.3 * capacity, 0.6 * capacity, 0.9 * capacity])
@@ -488,12 +539,33 @@ def main():
     if not args.no_cpu:
         cpu_s, scale, exact, reference = cpu_baseline(q, a, b, polymul.Plan, args.cpu_sample)
         result["cpu_baseline"] = {"value": round(cpu_s * scale * 1000.0, 1), "unit": "ms/product", "cores": 1,
-                                  "kind": "port",
+                                  "kind": "port", "extrapolated": True,
                                   "sample": "GMP Kronecker product (oracle, 1 core) of the first %d coefficients "
                                             "as an N=%d product with the same q: %.2f s, scaled to N=%d by "
                                             "B*log2(B) of the packed integer sizes (x%.2f)"
                                             % (args.cpu_sample, args.cpu_sample, cpu_s, n, scale),
                                   "sample_gpu_exact": exact}
+        # full-size oracle products, timed: N=1024 (256-bit q) in this process,
+        # N=32768 (27,776-bit q) alone in a child process, then one such product
+        # per core on every core this process may use (the CPU's throughput)
+        sn, sw, sl, sq, sa, sb = workloads.seeded_case("small")
+        t0 = time.perf_counter()
+        reference.kronecker_gmp(sa, sb, sq)
+        small_ms = (time.perf_counter() - t0) * 1000.0
+        mid_alone, _ = _oracle_processes("mid", 1)
+        cores = len(os.sched_getaffinity(0))
+        workers = min(cores, 16)
+        mid_each, mid_wall = _oracle_processes("mid", workers)
+        timed = {"B-small_ms": round(small_ms, 2), "B-mid_ms": round(mid_alone[0] * 1000.0, 1)}
+        result["cpu_baseline"]["timed_full_products"] = timed
+        gpu_mid = result.get("other_configs", {}).get("B-mid", {})
+        result["cpu_baseline"]["throughput"] = {
+            "config": "B-mid (N=32768, L=868), one oracle product per core, all started together",
+            "processes": workers, "cores_available": cores, "cpu_model": cpu_model(),
+            "products_per_s": round(workers / mid_wall, 3),
+            "mean_product_s": round(float(np.mean(mid_each)), 2),
+            "gpu_products_per_s_device": round(1000.0 / gpu_mid["ms_device"], 1) if gpu_mid else None,
+            "gpu_products_per_s_host_vectors": round(1000.0 / gpu_mid["e2e_host_ms"], 1) if gpu_mid else None}
         # sampled bit-exactness of the timed full-size product (checker only)
         checks = []
         for i in (0, n - 1):

@@ -3,6 +3,8 @@
   libpolymul_amd.so     kernels.hip + api.cpp + serve.cpp + plan.cpp   (C ABI, include/polymul.h)
   libfherma_amd.so      fherma_shim.cpp linked to the above (include/fherma.h)
   solution              driver_main.cpp: the main.cpp-compatible case runner
+  libpm_calibrate.so    calibrate.hip: register-only INT8 MFMA and 32-bit multiply
+                        throughput kernels (the measured peaks beside the datasheet ones)
 
 Every translation unit is compiled for gfx950 only (hipcc treats the .cpp
 sources as HIP too), with packed-FP32 instructions disabled (the kernels are
@@ -25,6 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 LIB = PKG / "libpolymul_amd.so"
+CALIBRATE_LIB = PKG / "libpm_calibrate.so"
 FHERMA_LIB = PKG / "libfherma_amd.so"
 SOLUTION = PKG / "solution"
 
@@ -57,6 +60,9 @@ def build(verbose=False):
         if driver.is_file():
             _run([HIPCC] + COMMON + DEVICE + [driver, "-L", PKG, "-lfherma_amd", "-lpolymul_amd",
                                      "-Wl,-rpath,$ORIGIN", "-o", SOLUTION], verbose)
+    calibrate = CSRC / "calibrate.hip"
+    if calibrate.is_file():
+        _run([HIPCC] + COMMON + DEVICE + ["-x", "hip", "-shared", calibrate, "-o", CALIBRATE_LIB], verbose)
     return LIB
 
 

@@ -12,12 +12,13 @@ thread_local std::string g_last_error;
 
 namespace {
 
-pm_status create(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, pm_plan** out, bool device) {
+pm_status create(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, pm_plan** out, bool device,
+                 uint32_t min_block = 1) {
     return guarded([&] {
         if (!out) throw std::invalid_argument("null output pointer");
         *out = nullptr;
         auto plan = std::make_unique<pm_plan>();
-        plan->host = pm::build_plan(n, width, limbs, q, device ? pm::Tables::kDevice : pm::Tables::kHost);
+        plan->host = pm::build_plan(n, width, limbs, q, device ? pm::Tables::kDevice : pm::Tables::kHost, min_block);
         if (device) plan->make_device();
         *out = plan.release();
     });
@@ -43,6 +44,11 @@ pm_status pm_plan_create(uint32_t n, uint32_t width, uint32_t limbs, const uint3
 
 pm_status pm_plan_create_host(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, pm_plan** out) {
     return create(n, width, limbs, q, out, false);
+}
+
+pm_status pm_plan_create_split(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, uint32_t min_block,
+                               int device, pm_plan** out) {
+    return create(n, width, limbs, q, out, device != 0, min_block);
 }
 
 void pm_plan_destroy(pm_plan* plan) { delete plan; }
@@ -256,7 +262,17 @@ pm_status pm_debug_stage(pm_plan* plan, const uint32_t* a, const uint32_t* b, in
         } else {
             // device rows have stride n_pad; the tap returns rows of exactly N
             const size_t elem = stage == PM_STAGE_WORDS ? 8 : 4;
-            if (stage == PM_STAGE_WORDS) {
+            if (stage == PM_STAGE_WORDS && pm::words_by_coefficient(s)) {
+                // coefficient-major on the device: transpose into the tap's [crt_words][N]
+                const size_t ws = pm::word_stride(s);
+                std::vector<int64_t> sums(size_t(s.n) * ws);
+                hip_check(hipMemcpyAsync(sums.data(), plan->work.words, sums.size() * 8, hipMemcpyDeviceToHost, st),
+                          "copy stage");
+                hip_check(hipStreamSynchronize(st), "debug stage");
+                int64_t* w = static_cast<int64_t*>(out);
+                for (size_t j = 0; j < s.crt_words; ++j)
+                    for (size_t i = 0; i < s.n; ++i) w[j * s.n + i] = sums[i * ws + j];
+            } else if (stage == PM_STAGE_WORDS) {
                 hip_check(hipMemcpy2DAsync(out, s.n * elem, plan->work.words, size_t(s.n_pad) * elem, s.n * elem,
                                            s.crt_words, hipMemcpyDeviceToHost, st),
                           "copy stage");

@@ -342,12 +342,18 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
 
 // ---------------------------------------------------------------------------
 // 5. CRT GEMM: G[c][i] = sum_{k,d} w[c][4k+d] * (byte_d(v_{k,i}) - 128),
-//    words[j][i] = sum_t 256^t G[4j+t][i]   (int64, exact)
+//    D_j(i) = sum_t 256^t G[4j+t][i]   (int64, exact)
+//    stored word-major words[j][i] (row stride n_pad), or, for the
+//    wave-per-coefficient reconstruction, coefficient-major words[i][j] (row
+//    stride ws): a wave's 4 x 16 words of 16 coefficients are then 128-byte
+//    runs per coefficient, written one after another.
 // ---------------------------------------------------------------------------
+template <bool kByCoefficient>
 __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtiles,
                                                 const int8_t* __restrict__ rtiles,
                                                 int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
-                                                uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words) {
+                                                uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words,
+                                                uint32_t ws) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
@@ -365,15 +371,28 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
     gemm_mainloop(a_panel, bl, ksteps, lds, acc);
 
     const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tn * kTile + wn * kWaveCols;
-#pragma unroll
-    for (uint32_t mi = 0; mi < kWaveM; ++mi) {
-        const uint32_t word = (m0 + mi * 16) / 4 + (lane >> 4);
-        if (word >= crt_words) continue;
-        int64_t* out = words + size_t(word) * n_pad;
+    if constexpr (kByCoefficient) {
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni) {
             const uint32_t coef = c0 + ni * 16 + (lane & 15);
-            if (coef < n) out[coef] = combine4(acc[mi][ni]);
+            int64_t* out = words + size_t(coef) * ws;
+#pragma unroll
+            for (uint32_t mi = 0; mi < kWaveM; ++mi) {
+                const uint32_t word = (m0 + mi * 16) / 4 + (lane >> 4);
+                if (word < crt_words && coef < n) out[word] = combine4(acc[mi][ni]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (uint32_t mi = 0; mi < kWaveM; ++mi) {
+            const uint32_t word = (m0 + mi * 16) / 4 + (lane >> 4);
+            if (word >= crt_words) continue;
+            int64_t* out = words + size_t(word) * n_pad;
+#pragma unroll
+            for (uint32_t ni = 0; ni < kWaveN; ++ni) {
+                const uint32_t coef = c0 + ni * 16 + (lane & 15);
+                if (coef < n) out[coef] = combine4(acc[mi][ni]);
+            }
         }
     }
 }
@@ -969,6 +988,145 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     }
 }
 
+// ---------------------------------------------------------------------------
+// 6'. reconstruction, one wave per coefficient (q wider than 62 limbs): the
+// coefficient's CRT word sums are coefficient-major (words[i][0 .. L+3]) and
+// each lane takes one of 64 consecutive words per round, so every round is one
+// 512-byte load and one 256-byte store, and N waves (not N lanes) keep the
+// memory system busy.
+//
+//   K  = floor((E + 2^63) / 2^64), E = sum_j 2^32j D_{L+1+j} + corr_e
+//   x_j = D_j + corr_t_j + K*negp_j - (Qe*q)_j            (|x_j| < 2^56)
+//   R  = sum_j x_j 2^32j = T - Qe*q
+// Qe is the Barrett quotient of T estimated from the four words of x from
+// m = floor(s/32) - 1 up (all words from 0 when s < 64): the words below m
+// shift that estimate by under 2^25 * 2^32m, i.e. under 2^-7 at the 2^s
+// scale, so Ts is off by at most one and Qe lies in [Q-2, Q+1].  One pass
+// normalizes x into 32-bit words (carry out of each round passed to the next)
+// and compares R with q on the way; in the rare case R < 0 or R >= q, Qe is
+// moved by one and the pass repeats, rewriting c.
+// ---------------------------------------------------------------------------
+struct WaveReconstructArgs {
+    uint32_t n, ws, limbs, t_words, i_begin, i_end;
+    uint32_t top_m, top_e, k_shift;
+    uint64_t mu;
+};
+
+__device__ __forceinline__ int32_t from_lane_below(int32_t v, uint32_t lane) {
+    const int32_t u = __shfl_up(v, 1, 64);
+    return lane ? u : 0;
+}
+
+__device__ __forceinline__ int64_t read_lane64(int64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(v)), l);
+    const uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(uint64_t(v) >> 32)), l);
+    return int64_t((uint64_t(hi) << 32) | lo);
+}
+
+// Normalizes one round of 64 signed words x (lane j: |x_j| < 2^57) plus the
+// wave-uniform carry into word 0 into 32-bit words; returns lane j's word and
+// leaves in `carry` the carry into the next round's word 0.
+__device__ __forceinline__ uint32_t normalize_round(int64_t x, int64_t& carry, uint32_t lane) {
+    if (lane == 0) x += carry;
+    const int32_t hi = int32_t(x >> 32);                              // |hi| < 2^25
+    const int64_t v = int64_t(uint32_t(x)) + from_lane_below(hi, lane);  // (-2^25, 2^32 + 2^25)
+    const int32_t cj = int32_t(v >> 32);                              // -1, 0 or 1
+    const int64_t u = int64_t(uint32_t(v)) + from_lane_below(cj, lane);  // [-1, 2^32]
+    const int32_t hi63 = __builtin_amdgcn_readlane(hi, 63), c63 = __builtin_amdgcn_readlane(cj, 63);
+    uint32_t w = uint32_t(u);
+    int64_t ripple = 0;
+    if (__ballot(u < 0 || u > int64_t(0xFFFFFFFFu))) {
+        // a word of -1 or 2^32 (rare): ripple its carry up the round
+        int64_t r = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int64_t t = read_lane64(u, l) + r;
+            if (lane == uint32_t(l)) w = uint32_t(t);
+            r = t >> 32;
+        }
+        ripple = r;
+    }
+    carry = int64_t(hi63) + c63 + ripple;
+    return w;
+}
+
+__global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restrict__ words, uint32_t* __restrict__ c,
+                                                         const uint32_t* __restrict__ q,
+                                                         const uint32_t* __restrict__ neg_p,
+                                                         const uint32_t* __restrict__ corr_t,
+                                                         const uint32_t* __restrict__ corr_e,
+                                                         WaveReconstructArgs args) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i = args.i_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= args.i_end) return;  // whole waves
+    const uint32_t L = args.limbs, tw = args.t_words;
+    const int64_t* d = words + size_t(i) * args.ws;
+    uint32_t* out = c + size_t(i) * L;
+
+    // K = floor((E + 2^63) / 2^64)
+    uint32_t ew[4];
+    {
+        int64_t carry = 0;
+        for (uint32_t j = 0; j < 4; ++j) {
+            const int64_t x = (j < 3 ? d[L + 1 + j] : 0) + int64_t(corr_e[j]) + carry;
+            ew[j] = uint32_t(uint64_t(x));
+            carry = x >> 32;
+        }
+    }
+    const int64_t kq = int64_t(uint64_t(ew[2]) + (uint64_t(ew[3]) << 32) + ((uint64_t(ew[1]) + 0x80000000ull) >> 32));
+
+    // Barrett estimate from the top four words: S = sum_t x_{m+t} 2^32t mod 2^128
+    // equals (T - [words below m]) / 2^32m exactly (|S| < 2^126).
+    uint64_t qe;
+    {
+        unsigned __int128 sum = 0;
+        for (uint32_t t = 0; t < 4; ++t) {
+            const uint32_t j = args.top_m + t;
+            if (j >= tw) break;
+            const int64_t x = (j <= L ? d[j] : 0) + int64_t(corr_t[j]) + (j < L ? kq * int64_t(neg_p[j]) : 0);
+            sum += (unsigned __int128)(__int128)x << (32 * t);
+        }
+        __int128 ts = __int128(sum) >> args.top_e;
+        if (ts < 0) ts = 0;
+        qe = uint64_t(((unsigned __int128)uint64_t(ts) * args.mu) >> args.k_shift);
+    }
+
+    for (int pass = 0; pass < 6; ++pass) {
+        const uint64_t q0 = uint32_t(qe), q1 = uint32_t(qe >> 32);
+        int64_t carry = 0;
+        int cmp = 0;  // sign of R - q over the words so far (decided by the highest differing word)
+        for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            int64_t x = 0;
+            uint32_t qj = 0;
+            if (j < tw) {
+                qj = j < L ? q[j] : 0;
+                const uint32_t qm1 = j >= 1 && j - 1 < L ? q[j - 1] : 0;
+                const uint32_t qm2 = j >= 2 && j - 2 < L ? q[j - 2] : 0;
+                // word j of Qe*q: lo(q0 q_j) + hi(q0 q_{j-1}) + lo(q1 q_{j-1}) + hi(q1 q_{j-2})
+                const uint64_t pa = q0 * qj, pb = q0 * qm1, pc = q1 * qm1, pd = q1 * qm2;
+                const int64_t sub = int64_t(uint32_t(pa)) + int64_t(pb >> 32) + int64_t(uint32_t(pc)) + int64_t(pd >> 32);
+                x = (j <= L ? d[j] : 0) + int64_t(corr_t[j]) + (j < L ? kq * int64_t(neg_p[j]) : 0) - sub;
+            }
+            const uint32_t w = normalize_round(x, carry, lane);
+            if (j < L) out[j] = w;
+            const uint64_t differ = __ballot(w != qj);
+            if (differ) {
+                const int top = 63 - __clzll(differ);
+                cmp = uint32_t(__builtin_amdgcn_readlane(int(w), top)) > uint32_t(__builtin_amdgcn_readlane(int(qj), top))
+                          ? 1
+                          : -1;
+            }
+        }
+        if (carry < 0) {
+            --qe;  // R < 0
+        } else if (carry > 0 || cmp >= 0) {
+            ++qe;  // R >= q
+        } else {
+            break;
+        }
+    }
+}
+
 __global__ void fill_zero(uint32_t* __restrict__ c, size_t words) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < words) c[i] = 0;
@@ -1278,8 +1436,12 @@ void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t*
 
 void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream) {
     const uint32_t blocks = (s.crt_rows / kTile) * (s.n_pad / kTile);
-    hipLaunchKernelGGL(crt_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words, s.n,
-                       s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words);
+    if (words_by_coefficient(s))
+        hipLaunchKernelGGL(crt_gemm<true>, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words,
+                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s));
+    else
+        hipLaunchKernelGGL(crt_gemm<false>, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words,
+                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s));
     check_launch("crt_gemm");
 }
 
@@ -1287,6 +1449,16 @@ void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* wo
                         uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream) {
     if (!count) return;
     const uint32_t i_end = i_begin + count < s.n ? i_begin + count : s.n;
+    if (words_by_coefficient(s)) {
+        // Barrett window: the four words of x from m up (m = floor(s/32) - 1, or 0 below s = 64)
+        const uint32_t top_m = s.s_shift >= 64 ? s.s_shift / 32 - 1 : 0;
+        WaveReconstructArgs args{s.n, word_stride(s), s.limbs, s.t_words, i_begin, i_end,
+                                 top_m, s.s_shift - 32 * top_m, s.k_shift, s.mu};
+        hipLaunchKernelGGL(reconstruct_waves, dim3((i_end - i_begin + 3) / 4), dim3(256), 0, stream, words, c, t.q,
+                           t.neg_p, t.corr_t, t.corr_e, args);
+        check_launch("reconstruct_waves");
+        return;
+    }
     ReconstructArgs args{s.n, s.n_pad, s.limbs, s.t_words, i_begin, i_end, s.s_shift, s.k_shift, s.mu};
     dim3 grid((count + 63) / 64);
     hipLaunchKernelGGL(reconstruct, grid, dim3(64), 0, stream, words, t_scratch, c, t.q, t.neg_p, t.corr_t,
@@ -1328,7 +1500,10 @@ void configure_kernels() {
         status = hipFuncSetAttribute(reinterpret_cast<const void*>(&conversion_gemm),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
         if (status == hipSuccess)
-            status = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm),
+            status = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm<false>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
+        if (status == hipSuccess)
+            status = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm<true>),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
     });
     if (status != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(status));

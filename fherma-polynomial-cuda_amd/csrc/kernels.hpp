@@ -27,9 +27,22 @@ struct DeviceWork {
     int8_t* packed = nullptr;    // [2][n_pad/256][conv_k/64][256][64] signed operand bytes
     uint32_t* planes = nullptr;  // [2][r_pad][n_pad] residues of a and b
     int8_t* rtiles = nullptr;    // CRT operand tiles of v_k (aliases the b plane, dead after the row stages)
-    int64_t* words = nullptr;    // [crt_words][n_pad] CRT word sums
-    uint32_t* t_words = nullptr; // [t_words][n_pad] T scratch (aliases the a plane, dead after the CRT GEMM)
+    int64_t* words = nullptr;    // CRT word sums: [crt_words][n_pad], or [n_pad][ws] (words_by_coefficient)
+    uint32_t* t_words = nullptr; // [t_words][n_pad] T scratch of the lane-per-coefficient reconstruction
+                                 // (aliases the a plane, dead after the CRT GEMM)
 };
+
+// Layout of the CRT word sums.  Up to 62 limbs (t_words <= 64) one lane
+// reconstructs one coefficient and the sums are word-major [crt_words][n_pad];
+// wider q takes one wave per coefficient over coefficient-major sums
+// [n_pad][ws], ws = crt_words rounded up to 16 (128-byte rows).
+inline bool words_by_coefficient(const Shape& s) { return s.t_words > 64; }
+inline uint32_t word_stride(const Shape& s) { return words_by_coefficient(s) ? (s.crt_words + 15) / 16 * 16 : s.n_pad; }
+inline size_t word_sum_elements(const Shape& s) {
+    return words_by_coefficient(s) ? size_t(s.n_pad) * word_stride(s) : size_t(s.crt_words) * s.n_pad;
+}
+// Rows of the a plane the T scratch needs (the lane-per-coefficient reconstruction only).
+inline uint32_t scratch_rows(const Shape& s) { return words_by_coefficient(s) ? 0 : s.t_words; }
 
 // One-time kernel attributes (dynamic LDS limits); call before any launch.
 void configure_kernels();

@@ -381,12 +381,15 @@ bool prime_walk(uint64_t step, const Words& bound, std::vector<uint32_t>* primes
     }
 }
 
-HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words, Tables tables) {
+HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words, Tables tables,
+                    uint32_t min_block) {
     if (!n || (n & (n - 1)) || n >= (1u << 29))
         throw std::invalid_argument("invalid point: N must be a power of two below 2^29");
     if (!width || !limbs || limbs != (width + 31) / 32)
         throw std::invalid_argument("invalid point: L must equal ceil(W/32)");
     if (!q_words) throw std::invalid_argument("invalid point: missing modulus");
+    if (!min_block || (min_block & (min_block - 1)) || min_block > n || min_block > 2048)
+        throw std::invalid_argument("invalid split: the minimum block must be a power of two <= min(N, 2048)");
     HostPlan plan;
     plan.tables = tables;
     const bool host_tables = tables == Tables::kHost;
@@ -407,7 +410,7 @@ HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* 
     // ---- primes (solve.cu:905-913, generalized with the transform split) ----
     Words qm1 = sub(q, Words{1});
     Words bound = mul_word(mul(qm1, qm1), 4 * n);
-    uint32_t n1 = 1;
+    uint32_t n1 = min_block;
     bool found = false;
     for (; n1 <= n; n1 *= 2) {
         if (prime_walk(2 * uint64_t(n / n1), bound, &plan.primes)) {

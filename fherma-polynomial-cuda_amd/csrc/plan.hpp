@@ -82,8 +82,11 @@ struct HostPlan {
 // Validation of the point, as fherma_init does (solve.cu:903-904): N a power of
 // two below 2^29, L == ceil(W/32), q of exactly L limbs' storage, 1 <= q.
 // Throws std::invalid_argument / std::runtime_error with a message.
+// The transform split n1 is the smallest power of two >= min_block whose prime
+// family covers 4N(q-1)^2 (min_block = 1 is the plain choice; larger values pin
+// each base-case kernel at small sizes).
 HostPlan build_plan(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q_words,
-                    Tables tables = Tables::kHost);
+                    Tables tables = Tables::kHost, uint32_t min_block = 1);
 
 // fherma_init's prime walk, generalized: descending p = 1 (mod step) from
 // floor((2^31-2)/step)*step + 1, primes only, while prod(p) <= bound, failing

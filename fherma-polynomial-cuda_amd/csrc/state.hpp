@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "arena.hpp"
+#include "chunks.hpp"
 #include "copy_pool.hpp"
 #include "pinned_cache.hpp"
 #include "kernels.hpp"
@@ -45,10 +46,6 @@ constexpr int kEvents = 8;
 constexpr int kStages = 8;
 
 namespace pm {
-
-// The host-vector path moves operands in and results out in up to this many
-// coefficient chunks, overlapping copies, transfers and the per-coefficient stages.
-constexpr int kMaxChunks = 8;
 
 // HIP resources of one sequence of host-vector products: a compute stream and two
 // copy streams (all default priority), the chunk events, stage-timing events,
@@ -196,28 +193,7 @@ struct pm_plan {
         }
     }
 
-    // Coefficient chunks of the host pipeline, in whole 256-coefficient tiles:
-    // chunk ch covers rows [r0, r1).
-    struct Chunking {
-        uint32_t count, per;  // chunks, tiles per chunk
-        void rows(uint32_t ch, size_t n, size_t& r0, size_t& r1) const {
-            r0 = std::min<size_t>(size_t(ch) * per * 256, n);
-            r1 = std::min<size_t>(size_t(ch + 1) * per * 256, n);
-        }
-    };
-    // About one chunk per 6 MB of operand, 1 .. kMaxChunks: each chunk costs some
-    // tens of microseconds of copy-engine and event hand-offs, which small
-    // products feel, while large ones want the overlap of many chunks.
-    uint32_t chunk_limit() const {
-        const size_t bytes = size_t(host.shape.n) * host.shape.limbs * sizeof(uint32_t);
-        const size_t per = size_t(6) << 20;
-        return uint32_t(std::max<size_t>(1, std::min<size_t>(pm::kMaxChunks, (bytes + per - 1) / per)));
-    }
-    Chunking chunking(uint32_t limit) const {
-        const uint32_t ntiles = host.shape.n_pad / 256;
-        const uint32_t count = std::max(1u, std::min(ntiles, limit));
-        return Chunking{count, (ntiles + count - 1) / count};
-    }
+    pm::Chunking chunking() const { return pm::make_chunking(host.shape.n_pad, host.shape.n, host.shape.limbs); }
     // First half of a host-vector product.  Per coefficient chunk, the copy
     // workers move a and b (one job) from the caller's pageable vectors into
     // pinned staging as [a rows | b rows], one H2D carries both on a copy
@@ -252,52 +228,60 @@ struct pm_plan {
                 if (armed) pool->set_active(false);
             }
         } idle{pool};
-        const Chunking chunks = chunking(chunk_limit());
-        for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-            size_t r0, r1;
-            chunks.rows(ch, s.n, r0, r1);
-            // the chunk's rows of a, then (two operands) its rows of b, at word
-            // offset nops*r0*L in both staging and io_a
-            const size_t base = nops * r0 * L, rows = r1 - r0;
-            if (rows) {
-                const pm::CopyPool::Segment segs[2] = {{lane->pin_in + base, a + r0 * L, rows * row_bytes},
-                                                       {lane->pin_in + base + rows * L, b ? b + r0 * L : nullptr,
-                                                        rows * row_bytes}};
-                pool->copy(segs, nops);
-                hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
-                                         hipMemcpyHostToDevice, copy_in),
-                          "H2D inputs");
+        // A failure while queuing leaves steps already queued still running on
+        // buffers the next call would reuse: wait for them before rethrowing.
+        try {
+            const pm::Chunking chunks = chunking();
+            for (uint32_t ch = 0; ch < chunks.count; ++ch) {
+                size_t r0, r1;
+                chunks.rows(ch, s.n, r0, r1);
+                // the chunk's rows of a, then (two operands) its rows of b, at word
+                // offset nops*r0*L in both staging and io_a
+                const size_t base = nops * r0 * L, rows = r1 - r0;
+                if (rows) {
+                    const pm::CopyPool::Segment segs[2] = {{lane->pin_in + base, a + r0 * L, rows * row_bytes},
+                                                           {lane->pin_in + base + rows * L, b ? b + r0 * L : nullptr,
+                                                            rows * row_bytes}};
+                    pool->copy(segs, nops);
+                    hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
+                                             hipMemcpyHostToDevice, copy_in),
+                              "H2D inputs");
+                }
+                hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
+                uint32_t t0, tc;
+                chunks.tiles(ch, t0, tc);
+                if (tc) {
+                    // pack reads coefficient i at x + i*L: offset the bases so rows
+                    // r0.. land on the chunk's a and b blocks
+                    const uint32_t* xa = io_a + (nops == 2 ? r0 * L : 0);
+                    const uint32_t* xb = io_a + r1 * L;
+                    pm::launch_pack(s, xa, xb, work.packed, t0, tc, 0, nops, stream);
+                    pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
+                }
             }
-            hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
-            const uint32_t t0 = ch * chunks.per, tc = std::min(chunks.per, s.n_pad / 256 - t0);
-            if (tc) {
-                // pack reads coefficient i at x + i*L: offset the bases so rows
-                // r0.. land on the chunk's a and b blocks
-                const uint32_t* xa = io_a + (nops == 2 ? r0 * L : 0);
-                const uint32_t* xb = io_a + r1 * L;
-                pm::launch_pack(s, xa, xb, work.packed, t0, tc, 0, nops, stream);
-                pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
+            run_middle(stream, nops, bplanes);
+            const bool direct = pm::reconstruct_writes_once(s);
+            for (uint32_t ch = 0; ch < chunks.count; ++ch) {
+                size_t r0, r1;
+                chunks.rows(ch, s.n, r0, r1);
+                pm::launch_reconstruct(s, tables, work.words, work.t_words, direct ? lane->pin_out : io_c, uint32_t(r0),
+                                       uint32_t(r1 - r0), stream);
+                if (direct) {
+                    hip_check(hipEventRecord(lane->out_ready[ch], stream), "hipEventRecord");
+                    continue;
+                }
+                hip_check(hipEventRecord(lane->reconstructed[ch], stream), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(copy_out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
+                if (r1 > r0)
+                    hip_check(hipMemcpyAsync(lane->pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
+                                             hipMemcpyDeviceToHost, copy_out),
+                              "D2H c");
+                hip_check(hipEventRecord(lane->out_ready[ch], copy_out), "hipEventRecord");
             }
-        }
-        run_middle(stream, nops, bplanes);
-        const bool direct = pm::reconstruct_writes_once(s);
-        for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-            size_t r0, r1;
-            chunks.rows(ch, s.n, r0, r1);
-            pm::launch_reconstruct(s, tables, work.words, work.t_words, direct ? lane->pin_out : io_c, uint32_t(r0),
-                                   uint32_t(r1 - r0), stream);
-            if (direct) {
-                hip_check(hipEventRecord(lane->out_ready[ch], stream), "hipEventRecord");
-                continue;
-            }
-            hip_check(hipEventRecord(lane->reconstructed[ch], stream), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(copy_out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
-            if (r1 > r0)
-                hip_check(hipMemcpyAsync(lane->pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
-                                         hipMemcpyDeviceToHost, copy_out),
-                          "D2H c");
-            hip_check(hipEventRecord(lane->out_ready[ch], copy_out), "hipEventRecord");
+        } catch (...) {
+            lane->synchronize();
+            throw;
         }
         pending = true;  // only once every step is queued: a failure above leaves the plan usable
         idle.armed = false;  // the pool keeps spinning until multiply_finish
@@ -318,7 +302,7 @@ struct pm_plan {
             pm::CopyPool* pool;
             ~Idle() { pool->set_active(false); }
         } idle{lane->pool.get()};
-        const Chunking chunks = chunking(chunk_limit());
+        const pm::Chunking chunks = chunking();
         for (uint32_t ch = 0; ch < chunks.count; ++ch) {
             size_t r0, r1;
             chunks.rows(ch, s.n, r0, r1);
@@ -373,11 +357,11 @@ struct pm_plan {
         // planes: [2][r_pad][n_pad].  After the row stages the b half is dead and
         // receives the CRT operand tiles of v_k (exactly r_pad*n_pad words);
         // after the CRT GEMM the a half is dead and holds T ([L+2][n_pad]).
-        const size_t half = size_t(s.r_pad > s.t_words ? s.r_pad : s.t_words) * s.n_pad;
+        const size_t half = size_t(std::max(s.r_pad, pm::scratch_rows(s))) * s.n_pad;
         work.planes = alloc<uint32_t>(2 * half);
         work.rtiles = reinterpret_cast<int8_t*>(work.planes + half);
         work.t_words = work.planes;
-        work.words = alloc<int64_t>(size_t(s.crt_words) * s.n_pad);
+        work.words = alloc<int64_t>(pm::word_sum_elements(s));
         // pinned staging of the host entry points is sized here, with the rest
         // of the plan, not inside the first product
         lane->reserve(nl);
@@ -498,8 +482,8 @@ struct pm_operand {
     uint32_t* planes = nullptr;
     ~pm_operand() {
         if (planes) {
-            hipSetDevice(plan->device_id);
-            hipFree(planes);
+            (void)hipSetDevice(plan->device_id);
+            (void)hipFree(planes);
         }
     }
 };
@@ -518,9 +502,9 @@ inline size_t plan_footprint(const pm::HostPlan& h) {
     t += aligned_bytes(size_t(2) * s.conv_rows * s.conv_k) + 2 * aligned_bytes(r * s.n2 * 2 * 4) +
          aligned_bytes(size_t(s.crt_rows) * s.crt_k);
     t += 2 * aligned_bytes(size_t(s.limbs) * 4) + aligned_bytes(size_t(s.t_words) * 4) + aligned_bytes(4 * 4);
-    const size_t half = size_t(s.r_pad > s.t_words ? s.r_pad : s.t_words) * s.n_pad;
+    const size_t half = size_t(std::max(s.r_pad, pm::scratch_rows(s))) * s.n_pad;
     t += aligned_bytes(pm::packed_operand_bytes(s)) + aligned_bytes(2 * half * 4) +
-         aligned_bytes(size_t(s.crt_words) * s.n_pad * 8);
+         aligned_bytes(pm::word_sum_elements(s) * 8);
     return t;
 }
 

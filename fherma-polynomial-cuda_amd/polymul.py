@@ -84,6 +84,28 @@ class RequestTimes(ctypes.Structure):
 _lib = None
 
 
+def hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        maps = Path("/proc/self/maps").read_text()
+    except OSError:
+        return set()
+    return {line.split()[-1] for line in maps.splitlines() if "libamdhip64.so" in line.split()[-1]}
+
+
+def _bind_one_hip_runtime():
+    """The library links ROCm's HIP runtime by its SONAME (libamdhip64.so.7);
+    PyTorch's ROCm wheel ships its own copy and links it by the unversioned
+    name.  Opening the runtime under the unversioned name first makes both
+    resolve to one copy, whichever of the two is loaded first: if torch is
+    already loaded this returns its copy, otherwise ROCm's copy is registered
+    under both names and a later `import torch` reuses it."""
+    try:
+        ctypes.CDLL("libamdhip64.so", mode=ctypes.RTLD_GLOBAL)
+    except OSError:
+        pass  # the library's own NEEDED entry still finds a runtime
+
+
 def load_library(path=None):
     """Loads libpolymul_amd.so; raises (never falls back) when it is absent."""
     global _lib
@@ -92,13 +114,19 @@ def load_library(path=None):
     path = Path(path) if path else LIBRARY_PATH
     if not path.is_file():
         raise FileNotFoundError("%s is not built; run `python fherma-polynomial-cuda_amd/build.py`" % path)
+    _bind_one_hip_runtime()
     lib = ctypes.CDLL(str(path))
+    if len(hip_runtimes()) > 1:
+        raise RuntimeError("two HIP runtimes are loaded in this process (%s); the second sees no GPU"
+                           % ", ".join(sorted(hip_runtimes())))
     u32p = ctypes.POINTER(ctypes.c_uint32)
     vp = ctypes.c_void_p
     plan_pp = ctypes.POINTER(ctypes.c_void_p)
     sigs = {
         "pm_plan_create": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, plan_pp], ctypes.c_int),
         "pm_plan_create_host": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, plan_pp], ctypes.c_int),
+        "pm_plan_create_split": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, u32p, ctypes.c_uint32,
+                                  ctypes.c_int, plan_pp], ctypes.c_int),
         "pm_plan_destroy": ([vp], None),
         "pm_multiply": ([vp, vp, vp, vp], ctypes.c_int),
         "pm_multiply_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
@@ -144,14 +172,20 @@ def _u32(array):
 class Plan:
     """A parameter-only plan (fherma_init) for c = a*b mod (X^N+1, q)."""
 
-    def __init__(self, n, width, limbs, q_words, device=True):
+    def __init__(self, n, width, limbs, q_words, device=True, min_block=None):
+        """min_block: force the transform split to start at n1 >= min_block
+        (pm_plan_create_split); None takes the plain choice."""
         self.lib = load_library()
         q_arr, q_ptr = _u32(q_words)
         if q_arr.size != limbs:
             raise ValueError("q must have exactly L=%d limbs" % limbs)
         handle = ctypes.c_void_p()
-        create = self.lib.pm_plan_create if device else self.lib.pm_plan_create_host
-        _check(self.lib, create(n, width, limbs, q_ptr, ctypes.byref(handle)))
+        if min_block is not None:
+            status = self.lib.pm_plan_create_split(n, width, limbs, q_ptr, min_block, int(device), ctypes.byref(handle))
+        else:
+            create = self.lib.pm_plan_create if device else self.lib.pm_plan_create_host
+            status = create(n, width, limbs, q_ptr, ctypes.byref(handle))
+        _check(self.lib, status)
         self.handle = handle
         self.n, self.width, self.limbs = n, width, limbs
         self.device = device

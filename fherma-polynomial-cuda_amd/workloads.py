@@ -103,6 +103,61 @@ def operands_for(n, q_words, seed):
     return uniform_below(q_words, n, rng), uniform_below(q_words, n, rng)
 
 
+def log_uniform_int(rng, lo, hi):
+    import math
+
+    return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+
+def make_products(count, seed, n_choices, limb_range):
+    """Independent products [(n, limbs, q, a, b)], each with its own seeded
+    random odd q and uniform operands."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(n_choices[rng.integers(0, len(n_choices))])
+        limbs = log_uniform_int(rng, *limb_range)
+        prng = np.random.default_rng([seed, i])
+        q = random_modulus(limbs, prng)
+        out.append((n, limbs, q, uniform_below(q, n, prng), uniform_below(q, n, prng)))
+    return out
+
+
+POWERS = [4096, 8192, 16384, 32768, 65536]
+# The batched workload: 64 products, N in {2^12..2^16}, L log-uniform in
+# [16, 1747] limbs, own q each.  The served workload: 48 requests, L in
+# [32, 1747].  The resident-operand workload: 64 random a times one fixed b at
+# N = 65536, L = 55.
+BATCH = {"count": 64, "seed": 0x5EED0004, "limbs": (16, 1747)}
+SERVED = {"count": 48, "seed": 0x5EED0006, "limbs": (32, 1747)}
+RESIDENT = {"count": 64, "seed": 0x5EED0005, "n": 65536, "limbs": 55}
+
+
+def batch_products(count=None):
+    return make_products(BATCH["count"] if count is None else count, BATCH["seed"], POWERS, BATCH["limbs"])
+
+
+def served_products(count=None):
+    return make_products(SERVED["count"] if count is None else count, SERVED["seed"], POWERS, SERVED["limbs"])
+
+
+def resident_case(count=None):
+    """(n, limbs, q, b, [a_0 .. a_{count-1}])."""
+    n, limbs = RESIDENT["n"], RESIDENT["limbs"]
+    rng = np.random.default_rng(RESIDENT["seed"])
+    q = random_modulus(limbs, rng)
+    b = uniform_below(q, n, rng)
+    a_list = [uniform_below(q, n, rng) for _ in range(RESIDENT["count"] if count is None else count)]
+    return n, limbs, q, b, a_list
+
+
+def digest(c):
+    """SHA-256 of an [N][L] result as little-endian 32-bit words."""
+    import hashlib
+
+    return hashlib.sha256(np.ascontiguousarray(c, dtype="<u4").tobytes()).hexdigest()
+
+
 def write_point_dir(root, n, width, limbs, q_words, cases):
     """A case directory in the harness's layout (src/main.cpp): manifest.json,
     point/q.bin, cases/NNNNNN/{a,b}.bin; `cases` is a list of (a, b)."""

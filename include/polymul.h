@@ -62,6 +62,15 @@ pm_status pm_plan_create(uint32_t n, uint32_t width, uint32_t limbs, const uint3
 pm_status pm_plan_create_host(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q,
                               pm_plan** out);
 
+/* A plan whose transform split N = n1 * n2 starts at n1 >= min_block (a power
+ * of two <= min(N, 2048)); pm_plan_create is min_block = 1, and takes the
+ * smallest split whose prime family suffices.  device = 0 builds host tables
+ * only, like pm_plan_create_host.  No counterpart in the reference, whose
+ * transform never splits: it runs each base-case kernel (n1 = 2, 4, 8 and the
+ * generic n1 >= 16) at sizes where the oracle checks every output. */
+pm_status pm_plan_create_split(uint32_t n, uint32_t width, uint32_t limbs, const uint32_t* q, uint32_t min_block,
+                               int device, pm_plan** out);
+
 /* Releases everything the plan holds: replaces fherma_free (src/fherma.h:48). */
 void pm_plan_destroy(pm_plan* plan);
 

@@ -1,0 +1,136 @@
+"""TEST INFRASTRUCTURE ONLY: full-output digests of the large workloads, made
+once with the CPU oracle (GMP Kronecker product, tests/reference.py:29-43
+restated in oracle/kronecker.c) and committed as tests/golden/workload_digests.json.
+
+The products are too large to recompute on every test run (the 458 MB-per-
+operand plain product takes about two minutes of GMP time and ~12 GB of RAM),
+so the GPU tests regenerate the same seeded inputs (workloads.py) and compare
+SHA-256 digests of the whole result:
+
+  large     N=65536, L=1747 (55,904-bit random odd q): digest of c, plus one
+            digest per block of 1024 coefficients to localize a mismatch
+  batch     the 64 products of the batched workload (one digest each)
+  served    the 48 requests of the served workload
+  resident  the 64 products a_i * b of the resident-operand workload
+  stress    the reference's fresh-seed stress cases (tools/stress_local.py:16-21):
+            seeds 10001-10020 at (N, W bits) = (32768, 868) and (65536, 1747),
+            10001-10004 at (128, 4096) and (131072, 128); q the NTT-friendly
+            prime below 2^W and the seeded pair of tests/reference.py:58-60
+
+Digests of the inputs are recorded too, so that a generator difference (a
+different numpy) is told apart from a wrong product.
+
+Usage:  python tests/golden/make_workload_golden.py [--jobs 4] [--only large,batch,...]
+"""
+import argparse
+import hashlib
+import importlib.util
+import json
+import sys
+import time
+from concurrent.futures import ProcessPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+OUT = Path(__file__).resolve().parent / "workload_digests.json"
+
+
+def _workloads():
+    spec = importlib.util.spec_from_file_location("pm_workloads", ROOT / "fherma-polynomial-cuda_amd" / "workloads.py")
+    module = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(module)
+    return module
+
+
+def _sha(x):
+    return hashlib.sha256(np.ascontiguousarray(x, dtype="<u4").tobytes()).hexdigest()
+
+
+def _product(args):
+    from oracle import reference
+
+    n, limbs, q, a, b = args
+    t0 = time.perf_counter()
+    c = reference.kronecker_gmp(a, b, q)
+    return {"n": n, "limbs": limbs, "q": _sha(q), "a": _sha(a), "b": _sha(b), "c": _sha(c),
+            "oracle_s": round(time.perf_counter() - t0, 2)}
+
+
+def large():
+    from oracle import reference
+
+    w = _workloads()
+    n, width, limbs, q, a, b = w.seeded_case("large")
+    t0 = time.perf_counter()
+    c = reference.kronecker_gmp(a, b, q)
+    seconds = time.perf_counter() - t0
+    return {"n": n, "limbs": limbs, "seed": w.CONFIGS["large"]["seed"], "q": _sha(q), "a": _sha(a), "b": _sha(b),
+            "c": _sha(c), "block": 1024, "blocks": [_sha(c[i:i + 1024]) for i in range(0, n, 1024)],
+            "oracle_s": round(seconds, 1)}
+
+
+STRESS = [(32768, 868, range(10001, 10021)), (65536, 1747, range(10001, 10021)),
+          (128, 4096, range(10001, 10005)), (131072, 128, range(10001, 10005))]
+
+
+def _stress_case(args):
+    from oracle import reference
+
+    n, width, seed = args
+    q = reference.modulus(n, width)
+    limbs = (width + 31) // 32
+    a_list, b_list = reference.random_pair(n, q, seed)
+    a, b = reference.to_limbs(a_list, limbs), reference.to_limbs(b_list, limbs)
+    qw = reference.int_to_words(q, limbs)
+    c = reference.kronecker_gmp(a, b, qw)
+    return {"n": n, "width": width, "seed": seed, "q": _sha(qw), "a": _sha(a), "b": _sha(b), "c": _sha(c)}
+
+
+def stress(jobs):
+    cases = [(n, width, seed) for n, width, seeds in STRESS for seed in seeds]
+    with ProcessPoolExecutor(max_workers=jobs) as pool:
+        return list(pool.map(_stress_case, cases))
+
+
+def products(items, jobs):
+    # largest first so that the pool finishes evenly
+    order = sorted(range(len(items)), key=lambda i: -items[i][0] * items[i][1])
+    out = [None] * len(items)
+    with ProcessPoolExecutor(max_workers=jobs) as pool:
+        for i, res in zip(order, pool.map(_product, [items[i] for i in order])):
+            out[i] = res
+    return out
+
+
+def main():
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--jobs", type=int, default=4, help="oracle processes (each up to ~12 GB for the largest)")
+    parser.add_argument("--only", default="large,batch,served,resident,stress")
+    args = parser.parse_args()
+    w = _workloads()
+    data = json.loads(OUT.read_text()) if OUT.is_file() else {}
+    for part in args.only.split(","):
+        t0 = time.perf_counter()
+        if part == "large":
+            data["large"] = large()
+        elif part == "batch":
+            data["batch"] = {"seed": w.BATCH["seed"], "products": products(w.batch_products(), args.jobs)}
+        elif part == "served":
+            data["served"] = {"seed": w.SERVED["seed"], "products": products(w.served_products(), args.jobs)}
+        elif part == "resident":
+            n, limbs, q, b, a_list = w.resident_case()
+            data["resident"] = {"seed": w.RESIDENT["seed"],
+                                "products": products([(n, limbs, q, a, b) for a in a_list], args.jobs)}
+        elif part == "stress":
+            data["stress"] = stress(args.jobs)
+        else:
+            raise SystemExit("unknown part %r" % part)
+        OUT.write_text(json.dumps(data, indent=1) + "\n")
+        print("%s: %.1f s" % (part, time.perf_counter() - t0), flush=True)
+
+
+if __name__ == "__main__":
+    main()

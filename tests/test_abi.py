@@ -50,3 +50,23 @@ def test_last_error_is_thread_local_and_readable(polymul):
     status = lib.pm_plan_create_host(3, 7, 1, (ctypes.c_uint32 * 1)(97), ctypes.byref(handle))
     assert status == 1
     assert b"power of two" in lib.pm_last_error()
+
+
+def test_one_hip_runtime_in_either_import_order(tmp_path):
+    """Loading the library before or after `import torch` leaves one HIP runtime
+    mapped (PyTorch's wheel ships its own copy under the unversioned name; a
+    second copy in the process would see no GPU)."""
+    import sys
+
+    script = tmp_path / "order.py"
+    script.write_text(
+        "import importlib.util, sys\n"
+        "spec = importlib.util.spec_from_file_location('pm_polymul', %r)\n"
+        "m = importlib.util.module_from_spec(spec); spec.loader.exec_module(m)\n"
+        "if sys.argv[1] == 'lib-first':\n    m.load_library(); import torch\n"
+        "else:\n    import torch; m.load_library()\n"
+        "print(len(m.hip_runtimes()))\n" % str(PKG / "polymul.py"))
+    for order in ("lib-first", "torch-first"):
+        proc = subprocess.run([sys.executable, str(script), order], capture_output=True, text=True, timeout=600)
+        assert proc.returncode == 0, proc.stderr[-2000:]
+        assert proc.stdout.strip() == "1", (order, proc.stdout)

@@ -225,3 +225,30 @@ def test_large_random_modulus_properties(polymul, workloads, oracle):
         top = oracle.words_to_int(a[-1])
         assert oracle.words_to_int(shifted[0]) == (-top) % oracle.words_to_int(q)
         assert np.array_equal(plan.multiply(b, a), c)
+
+
+# q wider than 62 limbs takes the wave-per-coefficient reconstruction: edge
+# operands whose results sit next to a multiple of q (all q-1: every result is
+# a small integer or q minus one) take its correction pass; a 62-bit q stored
+# in 70 limbs starts its Barrett window at word 0.
+@pytest.mark.parametrize("n,limbs,kind", [(256, 100, "random"), (64, 130, "random"), (128, 70, "narrow")])
+def test_wide_q_reconstruction_edges(polymul, workloads, oracle, n, limbs, kind):
+    _announce("wide-q edges", n, limbs, kind)
+    rng = np.random.default_rng(n * 1000 + limbs)
+    if kind == "narrow":
+        q_int = (1 << 61) + 1235
+        q = oracle.int_to_words(q_int, limbs)
+        draw = random.Random(limbs)
+        a = oracle.to_limbs([draw.randrange(q_int) for _ in range(n)], limbs)
+        b = oracle.to_limbs([draw.randrange(q_int) for _ in range(n)], limbs)
+    else:
+        q = workloads.random_modulus(limbs, rng)
+        a = workloads.uniform_below(q, n, rng)
+        b = workloads.uniform_below(q, n, rng)
+    with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+        _check_product(plan, a, b, q, oracle)
+        for kind_a in ("zero", "unit", "max", "x^%d" % (n - 1)):
+            e = _edge_operands(workloads, q, n, kind_a)
+            _check_product(plan, e, b, q, oracle)
+        m = _edge_operands(workloads, q, n, "max")
+        _check_product(plan, m, m, q, oracle)

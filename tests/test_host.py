@@ -36,3 +36,47 @@ def test_device_arena_with_a_fake_device(tmp_path):
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert proc.returncode == 0, proc.stdout[-2000:] + proc.stderr[-2000:]
     assert proc.stdout.strip() == "ok"
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ to build the chunk check")
+def test_pipeline_chunks_stay_inside_the_tiles(tmp_path):
+    """Every chunk of the host pipeline covers a non-empty tile range inside
+    [0, ntiles), for every power-of-two N up to 2^24 and L up to 8192 limbs
+    (tests/chunk_check.cpp).  A chunk starting past the last tile used to make
+    the packing kernel write past the packed-operand buffer (e.g. N=2048,
+    L=4096: 6 chunks of 2 tiles over 8 tiles)."""
+    root = Path(__file__).resolve().parent.parent
+    csrc = root / "fherma-polynomial-cuda_amd" / "csrc"
+    exe = tmp_path / "chunk_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(csrc), str(root / "tests" / "chunk_check.cpp"),
+                    "-o", str(exe)], check=True, capture_output=True, timeout=300)
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 0, proc.stdout[-2000:]
+    assert proc.stdout.startswith("ok ")
+
+
+@pytest.mark.skipif(shutil.which("cmake") is None or not Path("/opt/rocm/bin/hipcc").is_file(),
+                    reason="needs cmake and hipcc")
+def test_harness_cmake_build(tmp_path):
+    """The FHERMA runner's own build command (src/fherma.toml:11-20) on a copy
+    of the sources: `cmake -B build -DCMAKE_BUILD_TYPE=Release && cmake --build
+    build` must produce build/solution and the three harness entry points."""
+    root = Path(__file__).resolve().parent.parent
+    src = tmp_path / "solution_src"
+    (src / "fherma-polynomial-cuda_amd").mkdir(parents=True)
+    shutil.copy(root / "CMakeLists.txt", src)
+    shutil.copytree(root / "include", src / "include")
+    shutil.copytree(root / "fherma-polynomial-cuda_amd" / "csrc", src / "fherma-polynomial-cuda_amd" / "csrc")
+    proc = subprocess.run("cmake -B build -DCMAKE_BUILD_TYPE=Release && cmake --build build -j 8", shell=True,
+                          cwd=src, capture_output=True, text=True, timeout=900)
+    assert proc.returncode == 0, proc.stdout[-3000:] + proc.stderr[-3000:]
+    solution = src / "build" / "solution"
+    assert solution.is_file()
+    out = subprocess.run(["nm", "-D", "--defined-only", str(src / "build" / "libfherma_amd.so")],
+                         capture_output=True, text=True, check=True).stdout
+    names = subprocess.run(["c++filt"], input=out, capture_output=True, text=True, check=True).stdout
+    for signature in ("fherma_init(fherma::Point const&)", "fherma_run(void*, fherma::Inputs const&)",
+                      "fherma_free(void*)"):
+        assert signature in names, signature
+    usage = subprocess.run([str(solution)], capture_output=True, text=True)
+    assert usage.returncode == 2 and "usage" in usage.stderr

@@ -4,6 +4,7 @@ closed form computed here with Python integers, and the whole arithmetic of
 the GPU path is replayed with numpy from the exported tables alone and
 compared limb-for-limb with the oracle."""
 import json
+import random
 import shutil
 import subprocess
 from pathlib import Path
@@ -285,3 +286,177 @@ def test_invalid_points_and_host_only_plans(polymul, oracle):
         with pytest.raises(polymul.PolymulError) as err:
             plan.multiply(a, a)
         assert err.value.status == 6  # PM_ERR_NO_DEVICE
+
+
+def test_forced_split_plans(polymul, oracle, workloads):
+    """pm_plan_create_split: the split starts at the requested block, its primes
+    are the reference's walk with step 2*n2, and bad minimum blocks are refused."""
+    n, limbs = 4096, 8
+    q = _random_q(workloads, limbs, 4242)
+    qw = oracle.int_to_words(q, limbs)
+    for block in (1, 2, 4, 8, 16, 64, 2048):
+        with polymul.Plan(n, 32 * limbs, limbs, qw, device=False, min_block=block) as plan:
+            info = plan.info()
+            assert info["block"] == block and info["transform"] == n // block
+            assert info["reference_walk"] == (1 if block == 1 else 0)
+            primes = [int(p) for p in plan.primes()]
+            assert primes == oracle.reference_primes(n, q, step=2 * (n // block))
+    for bad in (0, 3, 8192, 4096):  # not a power of two, above N, above the 2048-wide row
+        with pytest.raises(polymul.PolymulError) as err:
+            polymul.Plan(n, 32 * limbs, limbs, qw, device=False, min_block=bad)
+        assert err.value.status == polymul.PM_ERR_INVALID_ARGUMENT
+
+
+def _normalize_round(x, carry):
+    """The kernel's round normalization (kernels.hip normalize_round) in Python:
+    x is the list of 64 signed lane words, carry the carry into word 0.  Returns
+    (32-bit words, carry into the next round, whether the rare ripple ran)."""
+    M = (1 << 32) - 1
+    x = list(x)
+    x[0] += carry
+    hi = [v >> 32 for v in x]
+    v = [(x[j] & M) + (hi[j - 1] if j else 0) for j in range(64)]
+    c = [t >> 32 for t in v]
+    u = [(v[j] & M) + (c[j - 1] if j else 0) for j in range(64)]
+    assert all(abs(h) < (1 << 25) for h in hi) and all(t in (-1, 0, 1) for t in c)
+    ripple, slow = 0, any(t < 0 or t > M for t in u)
+    w = [t & M for t in u]
+    if slow:
+        r = 0
+        for j in range(64):
+            t = u[j] + r
+            w[j] = t & M
+            r = t >> 32
+        ripple = r
+    return w, hi[63] + c[63] + ripple, slow
+
+
+def _wave_reconstruct(d, limbs, q, neg_p, corr_t, corr_e, s_shift, k_shift, mu, bias=0):
+    """reconstruct_waves for one coefficient: d is its list of CRT word sums
+    D_0 .. D_{L+3}.  Returns (c as an int, passes taken).  `bias` moves the
+    Barrett estimate (tests of the correction passes)."""
+    L, tw = limbs, limbs + 2
+    qw = [(q >> (32 * j)) & 0xFFFFFFFF for j in range(L)]
+    nw = [(neg_p >> (32 * j)) & 0xFFFFFFFF for j in range(L)]
+    cw = [(corr_t >> (32 * j)) & 0xFFFFFFFF for j in range(tw)]
+    e = sum(d[L + 1 + j] << (32 * j) for j in range(3)) + corr_e
+    kq = (e + (1 << 63)) >> 64
+    top_m = s_shift // 32 - 1 if s_shift >= 64 else 0
+    top_e = s_shift - 32 * top_m
+
+    def base(j):
+        return (d[j] if j <= L else 0) + cw[j] + (kq * nw[j] if j < L else 0)
+
+    total = sum(base(top_m + t) << (32 * t) for t in range(4) if top_m + t < tw) % (1 << 128)
+    if total >= 1 << 127:
+        total -= 1 << 128
+    ts = max(total >> top_e, 0)
+    qe = max(((ts * mu) >> k_shift) + bias, 0)
+    for passes in range(1, 7):
+        q0, q1 = qe & 0xFFFFFFFF, qe >> 32
+        carry, cmp, out = 0, 0, []
+        for j0 in range(0, tw, 64):
+            xs, qs = [], []
+            for lane in range(64):
+                j = j0 + lane
+                if j >= tw:
+                    xs.append(0)
+                    qs.append(0)
+                    continue
+                qj = qw[j] if j < L else 0
+                qm1 = qw[j - 1] if 1 <= j <= L else 0
+                qm2 = qw[j - 2] if 2 <= j <= L + 1 else 0
+                sub = ((q0 * qj) & 0xFFFFFFFF) + ((q0 * qm1) >> 32) + ((q1 * qm1) & 0xFFFFFFFF) + ((q1 * qm2) >> 32)
+                x = base(j) - sub
+                assert abs(x) < (1 << 56)
+                xs.append(x)
+                qs.append(qj)
+            w, carry, _ = _normalize_round(xs, carry)
+            out.extend(w[:max(0, min(64, L - j0))])
+            differ = [lane for lane in range(64) if w[lane] != qs[lane]]
+            if differ:
+                top = differ[-1]
+                cmp = 1 if w[top] > qs[top] else -1
+        if carry < 0:
+            qe -= 1
+        elif carry > 0 or cmp >= 0:
+            qe += 1
+        else:
+            return sum(x << (32 * j) for j, x in enumerate(out)), passes
+    raise AssertionError("no convergence")
+
+
+@pytest.mark.parametrize("n,width,kind", [(16, 32 * 70, "random"), (8, 32 * 130, "random"), (8, 32 * 70, "small-q")])
+def test_wave_reconstruction_restatement(polymul, oracle, workloads, n, width, kind):
+    """The wave-per-coefficient reconstruction (q wider than 62 limbs) replayed
+    in Python from the plan's tables and the exact CRT word sums: normalization
+    in rounds of 64 words, the Barrett window of four words, and the correction
+    passes.  Edge operands (zero, all q-1, q//2) and a q far narrower than its
+    limb storage (s < 64: the window starts at word 0) are included; the result
+    must equal the oracle's product."""
+    limbs = (width + 31) // 32
+    if kind == "small-q":
+        q = (1 << 61) + 1235  # 62-bit q in 70 limbs of storage
+    else:
+        q = _random_q(workloads, limbs, n * 13 + limbs)
+    qw = oracle.int_to_words(q, limbs)
+    rng = np.random.Generator(np.random.PCG64(limbs))
+    if kind == "small-q":
+        draw = random.Random(n)
+        cases = [tuple(oracle.to_limbs([draw.randrange(q) for _ in range(n)], limbs) for _ in range(2))]
+    else:
+        cases = [(workloads.uniform_below(qw, n, rng), workloads.uniform_below(qw, n, rng))]
+    maxed = np.tile(workloads.q_minus_one(qw), (n, 1))
+    half = np.tile(oracle.int_to_words(q // 2, limbs), (n, 1))
+    cases += [(np.zeros_like(maxed), maxed), (maxed, maxed), (half, half + np.eye(1, limbs, dtype=np.uint32)[0])]
+    with _plan(polymul, oracle, n, width, q) as plan:
+        s = plan.shape()
+        assert s["t_words"] > 64
+        primes = [int(p) for p in plan.primes()]
+        product = 1
+        for p in primes:
+            product *= p
+        wc = plan.table(polymul.TABLE_CRT_WEIGHTS, np.int8).reshape(s["crt_rows"], s["crt_k"]).astype(np.int64)
+        tabs = dict(q=q, neg_p=oracle.words_to_int(plan.table(polymul.TABLE_NEG_P_MOD_Q, np.uint32)),
+                    corr_t=oracle.words_to_int(plan.table(polymul.TABLE_CORR_T, np.uint32)),
+                    corr_e=oracle.words_to_int(plan.table(polymul.TABLE_CORR_E, np.uint32)),
+                    s_shift=s["s_shift"], k_shift=s["k_shift"], mu=s["mu"])
+        for a, b in cases:
+            xs = oracle.signed_convolution(oracle.from_limbs(a), oracle.from_limbs(b))
+            vb = np.zeros((n, s["r_pad"]), dtype=np.uint32)
+            for k, p in enumerate(primes):
+                inv = pow((product // p) % p, -1, p)
+                vb[:, k] = [(xi % p) * inv % p for xi in xs]
+            v = vb.view(np.uint8).reshape(n, 4 * s["r_pad"]).astype(np.int64) - 128
+            g = wc @ v.T
+            sums = g[0::4] + 256 * g[1::4] + 65536 * g[2::4] + 16777216 * g[3::4]
+            runs = [_wave_reconstruct([int(sums[j, i]) for j in range(limbs + 4)], limbs, **tabs) for i in range(n)]
+            assert [r[0] for r in runs] == [xi % q for xi in xs]
+            if a is cases[0][0]:
+                # uniform operands: the four-word window estimate needs no correction (structured
+                # results within 2^-16 q of a multiple of q, e.g. all q-1, may take a second pass)
+                assert all(r[1] == 1 for r in runs)
+            # an estimate moved by -2 .. +2 is corrected by as many extra passes
+            for bias in (-2, -1, 1, 2):
+                d0 = [int(sums[j, 0]) for j in range(limbs + 4)]
+                value, passes = _wave_reconstruct(d0, limbs, bias=bias, **tabs)
+                assert value == xs[0] % q and passes <= abs(bias) + 1
+
+
+def test_wave_normalization_ripples():
+    """The rare path of a round: words of -1 and 2^32 generate carries that
+    ripple through runs of 0 and 2^32-1; the result must be the exact sum."""
+    rng = np.random.default_rng(5)
+    M = (1 << 32) - 1
+    for trial in range(300):
+        x = [int(v) for v in rng.integers(-(1 << 40), 1 << 40, size=64)]
+        # plant runs that make u = -1 / 2^32 and propagate
+        for j in rng.integers(0, 64, size=rng.integers(0, 6)):
+            x[int(j)] = int(rng.choice([0, M, -(1 << 32), 1 << 32, -1, M + 1]))
+        if trial % 3 == 0:
+            x = [int(rng.choice([0, M, -1, 1 << 32])) for _ in range(64)]
+        carry = int(rng.integers(-(1 << 24), 1 << 24))
+        w, out_carry, _ = _normalize_round(x, carry)
+        assert all(0 <= t <= M for t in w)
+        assert sum(t << (32 * j) for j, t in enumerate(w)) + (out_carry << (32 * 64)) == \
+            sum(t << (32 * j) for j, t in enumerate(x)) + carry
