@@ -94,6 +94,33 @@ pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b
     });
 }
 
+pm_status pm_multiply_device_scratch(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                                     void* stream) {
+    return guarded([&] {
+        require_device(plan);
+        require_idle(plan);
+        if (!a || !b || !c) throw std::invalid_argument("null operand");
+        hip_check(hipSetDevice(plan->device_id), "hipSetDevice");
+        const pm::Shape& s = plan->host.shape;
+        pm::DeviceWork w;
+        if (!s.trivial) {
+            const size_t need = pm::work_bytes(s);
+            if (need > pm::kScratchBytes)
+                throw std::invalid_argument("pm_multiply_device_scratch: the product needs " + std::to_string(need) +
+                                            " bytes of workspace, more than the " +
+                                            std::to_string(pm::kScratchBytes) + "-byte static scratch");
+            std::string why;
+            void* base = pm::scratch_address(plan->device_id, &why);
+            if (!base) throw DeviceError("pm_multiply_device_scratch: " + why);
+            pm::carve_work(s, static_cast<uint8_t*>(base), &w);
+        }
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plan->lane->stream;
+        plan->run_with(w, a, b, c, PM_STAGE_OUTPUT, st);
+        hip_check(hipStreamSynchronize(st), "multiply");
+        plan->collect_times();
+    });
+}
+
 pm_status pm_operand_create(pm_plan* plan, const uint32_t* b, pm_operand** out) {
     return guarded([&] {
         if (!out) throw std::invalid_argument("null output pointer");

@@ -1491,6 +1491,73 @@ void launch_build_tables(const Shape& s, const TableBuild& t, hipStream_t stream
     check_launch("table_crt");
 }
 
+// ---------------------------------------------------------------------------
+// Static workspace of the allocation-free device entry point
+// (pm_multiply_device_scratch).  The array is part of this code object: each
+// GPU that loads the library holds its own copy, at its own address, until
+// the process ends.  In host code `scratch` names only a host-side stand-in,
+// so its device address is looked up once per device (init_scratch, called
+// when the library makes a plan on that device) and only that address is
+// handed to kernels.
+// ---------------------------------------------------------------------------
+namespace {
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kScratchDevices] = {};
+hipError_t g_scratch_error[kScratchDevices] = {};
+}  // namespace
+
+void init_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) {
+        (void)hipGetLastError();
+        return;  // pm_multiply_device_scratch reports the device as unsupported
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();  // keep the failure out of later launch checks
+        g_scratch_error[dev] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        return;
+    }
+    g_scratch_ptr[dev] = ptr;
+}
+
+void* scratch_address(int device, std::string* why) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0 || device >= kScratchDevices) {
+        if (why) *why = "device " + std::to_string(device) + " is beyond the devices the static scratch supports";
+        return nullptr;
+    }
+    if (!g_scratch_ptr[device] && why)
+        *why = g_scratch_error[device] != hipSuccess
+                   ? std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_error[device])
+                   : "the static scratch was not set up on device " + std::to_string(device);
+    return g_scratch_ptr[device];
+}
+
+size_t work_bytes(const Shape& s) { return carve_work(s, nullptr, nullptr); }
+
+size_t carve_work(const Shape& s, uint8_t* base, DeviceWork* w) {
+    const size_t packed = packed_operand_bytes(s);
+    const size_t half = size_t(s.r_pad > scratch_rows(s) ? s.r_pad : scratch_rows(s)) * s.n_pad;
+    const size_t planes = 2 * half * sizeof(uint32_t);
+    const size_t words = word_sum_elements(s) * sizeof(int64_t);
+    auto round = [](size_t b) { return (b + 255) / 256 * 256; };
+    if (base && w) {
+        w->packed = reinterpret_cast<int8_t*>(base);
+        w->planes = reinterpret_cast<uint32_t*>(base + round(packed));
+        w->rtiles = reinterpret_cast<int8_t*>(w->planes + half);
+        w->t_words = w->planes;
+        w->words = reinterpret_cast<int64_t*>(base + round(packed) + round(planes));
+    }
+    return round(packed) + round(planes) + round(words);
+}
+
 void configure_kernels() {
     // The GEMMs use 64 KB of dynamic LDS (two stages of A and B tiles).  Plans
     // may be created on several threads at once (server workers): set once.

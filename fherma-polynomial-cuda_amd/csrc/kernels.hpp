@@ -3,9 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
+#include <string>
 
 #include "plan.hpp"
+#include "polymul.h"
 
 namespace pm {
 
@@ -46,6 +49,19 @@ inline uint32_t scratch_rows(const Shape& s) { return words_by_coefficient(s) ? 
 
 // One-time kernel attributes (dynamic LDS limits); call before any launch.
 void configure_kernels();
+
+// Static device workspace of pm_multiply_device_scratch: kScratchBytes per GPU,
+// held from the library's first plan on that GPU until the process ends.
+constexpr size_t kScratchBytes = PM_SCRATCH_BYTES;
+// Looks up the calling thread's current device's copy of the array (once per
+// device); a failure is recorded for scratch_address to report.
+void init_scratch();
+// The device address of `device`'s copy, or nullptr with the reason in *why.
+void* scratch_address(int device, std::string* why);
+// Working buffers of one device product carved from `base` in 256-byte units
+// (base == nullptr: size only); returns the bytes they span.
+size_t carve_work(const Shape& s, uint8_t* base, DeviceWork* w);
+size_t work_bytes(const Shape& s);
 size_t packed_operand_bytes(const Shape& s);
 
 // Stage launches, all on `stream`.

@@ -331,6 +331,7 @@ struct pm_plan {
         io_c = alloc<uint32_t>(nl);
         if (s.trivial) return;
         pm::configure_kernels();
+        pm::init_scratch();  // per-device address of pm_multiply_device_scratch's static workspace
         if (host.tables == pm::Tables::kDevice) {
             build_device_tables();
         } else {
@@ -419,12 +420,20 @@ struct pm_plan {
     }
 
     // The b residues inside the work planes ([r_pad][n_pad] after a's).
-    uint32_t* b_planes() { return work.planes + size_t(host.shape.r_pad) * host.shape.n_pad; }
+    uint32_t* b_planes() const { return b_planes(work); }
+    uint32_t* b_planes(const pm::DeviceWork& w) const { return w.planes + size_t(host.shape.r_pad) * host.shape.n_pad; }
 
     // Runs stages up to and including `last` (PM_STAGE_*) on device operands;
     // b == nullptr takes the b residues from `bplanes_in` (a resident operand).
     void run(const uint32_t* a, const uint32_t* b, uint32_t* c, int last, hipStream_t st,
              const uint32_t* bplanes_in = nullptr) {
+        run_with(work, a, b, c, last, st, bplanes_in);
+    }
+
+    // The same stages on the working buffers `w` (the plan's own, or the
+    // static workspace of pm_multiply_device_scratch).
+    void run_with(const pm::DeviceWork& w, const uint32_t* a, const uint32_t* b, uint32_t* c, int last,
+                  hipStream_t st, const uint32_t* bplanes_in = nullptr) {
         const pm::Shape& s = host.shape;
         if (s.trivial) {
             pm::launch_zero(c, size_t(s.n) * s.limbs, st);
@@ -435,24 +444,24 @@ struct pm_plan {
         };
         const uint32_t ntiles = s.n_pad / 256;
         const uint32_t nops = b ? 2 : 1;
-        const uint32_t* bplanes = b ? b_planes() : bplanes_in;
+        const uint32_t* bplanes = b ? b_planes(w) : bplanes_in;
         mark(0);
-        pm::launch_pack(s, a, b, work.packed, 0, ntiles, 0, nops, st);
+        pm::launch_pack(s, a, b, w.packed, 0, ntiles, 0, nops, st);
         mark(1);
-        pm::launch_conversion(s, tables, work.packed, work.planes, 0, ntiles, 0, nops, st);
+        pm::launch_conversion(s, tables, w.packed, w.planes, 0, ntiles, 0, nops, st);
         mark(2);
         if (last == PM_STAGE_RESIDUES) return;
-        pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
+        pm::launch_ntt_forward_columns(s, tables, w.planes, 0, nops, st);
         mark(3);
-        pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
+        pm::launch_ntt_rows_product(s, tables, w.planes, bplanes, st);
         mark(4);
-        pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
+        pm::launch_ntt_inverse_columns(s, tables, w.planes, w.rtiles, st);
         mark(5);
         if (last == PM_STAGE_PRODUCT) return;
-        pm::launch_crt(s, tables, work.rtiles, work.words, st);
+        pm::launch_crt(s, tables, w.rtiles, w.words, st);
         mark(6);
         if (last == PM_STAGE_WORDS) return;
-        pm::launch_reconstruct(s, tables, work.words, work.t_words, c, 0, s.n, st);
+        pm::launch_reconstruct(s, tables, w.words, w.t_words, c, 0, s.n, st);
         mark(7);
     }
 

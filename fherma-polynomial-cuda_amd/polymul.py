@@ -44,6 +44,8 @@ TABLE_RECIPROCALS = 10
 TABLE_BETA = 11
 TABLE_SHAPE = 12
 
+SCRATCH_BYTES = 128 << 20  # PM_SCRATCH_BYTES: static workspace of pm_multiply_device_scratch
+
 STAGE_RESIDUES = 1
 STAGE_PRODUCT = 2
 STAGE_WORDS = 3
@@ -130,6 +132,7 @@ def load_library(path=None):
         "pm_plan_destroy": ([vp], None),
         "pm_multiply": ([vp, vp, vp, vp], ctypes.c_int),
         "pm_multiply_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "pm_multiply_device_scratch": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "pm_multiply_begin": ([vp, vp, vp], ctypes.c_int),
         "pm_multiply_finish": ([vp, vp], ctypes.c_int),
         "pm_last_error": ([], ctypes.c_char_p),
@@ -270,6 +273,14 @@ class Plan:
         _check(self.lib, self.lib.pm_multiply_device(self.handle, ctypes.c_void_p(a_ptr), ctypes.c_void_p(b_ptr),
                                                      ctypes.c_void_p(c_ptr),
                                                      ctypes.c_void_p(stream) if stream else None))
+
+    def multiply_device_scratch(self, a_ptr, b_ptr, c_ptr, stream=None):
+        """multiply_device with the working buffers in the library's static
+        per-GPU workspace (pm_multiply_device_scratch): the call allocates
+        nothing; plans whose workspace exceeds SCRATCH_BYTES are refused."""
+        _check(self.lib, self.lib.pm_multiply_device_scratch(self.handle, ctypes.c_void_p(a_ptr),
+                                                             ctypes.c_void_p(b_ptr), ctypes.c_void_p(c_ptr),
+                                                             ctypes.c_void_p(stream) if stream else None))
 
     def debug_stage(self, a, b, stage):
         a, _ = _u32(a)

@@ -99,6 +99,25 @@ pm_status pm_multiply_finish(pm_plan* plan, uint32_t* c);
 pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c,
                              void* stream);
 
+/* pm_multiply_device without any allocation: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory in the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The product's working buffers (packed operands, residue planes, CRT word
+ * sums) come from a static workspace of PM_SCRATCH_BYTES (128 MiB) per GPU,
+ * which each GPU that loads the library holds until the process ends, instead
+ * of the plan's own buffers; the call makes no hipMalloc, hipFree or other
+ * allocating call.  A plan whose workspace exceeds PM_SCRATCH_BYTES is
+ * rejected with PM_ERR_INVALID_ARGUMENT: the reference's own point (N = 65536,
+ * 1747-bit q, 125.5 MiB) fits, N = 65536 with a q of 61 limbs or more does
+ * not.  Every call of this entry point on one GPU shares that GPU's workspace,
+ * which keeps the last call's contents: such calls that overlap on one GPU
+ * (from different threads, or queued on different streams) race and must be
+ * serialized by the caller.  The plan's other entry points use the plan's own
+ * buffers and do not touch the workspace. */
+#define PM_SCRATCH_BYTES ((size_t)128 << 20)
+pm_status pm_multiply_device_scratch(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                                     void* stream);
+
 /* Resident operand: no counterpart in the reference (its operands always
  * arrive as a pair).  For callers that multiply many a by one fixed b: b is
  * converted and transformed once and its residues stay on the device

@@ -286,6 +286,9 @@ def test_invalid_points_and_host_only_plans(polymul, oracle):
         with pytest.raises(polymul.PolymulError) as err:
             plan.multiply(a, a)
         assert err.value.status == 6  # PM_ERR_NO_DEVICE
+        with pytest.raises(polymul.PolymulError) as err:
+            plan.multiply_device_scratch(1, 1, 1)  # refused before any pointer is used
+        assert err.value.status == 6
 
 
 def test_forced_split_plans(polymul, oracle, workloads):
