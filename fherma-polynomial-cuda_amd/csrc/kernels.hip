@@ -353,13 +353,14 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
                                                 const int8_t* __restrict__ rtiles,
                                                 int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
                                                 uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words,
-                                                uint32_t ws) {
+                                                uint32_t ws, uint32_t tile0, uint32_t tiles) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
-    const uint32_t ntiles = n_pad / kTile, num_m = crt_rows / kTile;
+    const uint32_t num_m = crt_rows / kTile;
     uint32_t tm, tn;
-    tile_coords(num_m, ntiles, tm, tn);
+    tile_coords(num_m, tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles)
+    tn += tile0;
     const int8_t* a_panel = wtiles + size_t(tm) * ksteps * kTileBytes;
     TiledB bl{rtiles + size_t(tn) * ksteps * kTileBytes};
 
@@ -1434,14 +1435,18 @@ void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t*
     check_launch("ntt_columns_inverse_tiles");
 }
 
-void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream) {
-    const uint32_t blocks = (s.crt_rows / kTile) * (s.n_pad / kTile);
+void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream,
+                uint32_t tile0, uint32_t tiles) {
+    const uint32_t ntiles = s.n_pad / kTile;
+    if (tile0 >= ntiles || tiles == 0) return;
+    if (tiles > ntiles - tile0) tiles = ntiles - tile0;
+    const uint32_t blocks = (s.crt_rows / kTile) * tiles;
     if (words_by_coefficient(s))
         hipLaunchKernelGGL(crt_gemm<true>, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words,
-                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s));
+                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), tile0, tiles);
     else
         hipLaunchKernelGGL(crt_gemm<false>, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words,
-                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s));
+                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), tile0, tiles);
     check_launch("crt_gemm");
 }
 

@@ -81,7 +81,9 @@ void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* pl
 // [n_pad/256][r_pad/16][256][64] (signed bytes).
 void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
                                 hipStream_t stream);
-void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream);
+// Coefficient tiles [tile0, tile0 + tiles), clamped to the plan's (default: all).
+void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream,
+                uint32_t tile0 = 0, uint32_t tiles = ~0u);
 // Coefficients [i_begin, i_begin + count), i_begin a multiple of 64.  When
 // reconstruct_writes_once(s), c is only written (each word once), so it may be
 // pinned host memory; otherwise c is read back and must be device memory.

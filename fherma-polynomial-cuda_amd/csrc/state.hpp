@@ -260,11 +260,17 @@ struct pm_plan {
                     pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
                 }
             }
-            run_middle(stream, nops, bplanes);
+            run_transforms(stream, nops, bplanes);
+            // Output half, per chunk: the CRT GEMM of the chunk's coefficient
+            // tiles, its reconstruction, then its D2H on the copy stream, which
+            // overlaps the next chunk's GEMM.
             const bool direct = pm::reconstruct_writes_once(s);
             for (uint32_t ch = 0; ch < chunks.count; ++ch) {
                 size_t r0, r1;
                 chunks.rows(ch, s.n, r0, r1);
+                uint32_t t0, tc;
+                chunks.tiles(ch, t0, tc);
+                pm::launch_crt(s, tables, work.rtiles, work.words, stream, t0, tc);
                 pm::launch_reconstruct(s, tables, work.words, work.t_words, direct ? lane->pin_out : io_c, uint32_t(r0),
                                        uint32_t(r1 - r0), stream);
                 if (direct) {
@@ -465,13 +471,13 @@ struct pm_plan {
         mark(7);
     }
 
-    // The stages between the per-coefficient ends: transforms and the CRT GEMM.
-    void run_middle(hipStream_t st, uint32_t nops, const uint32_t* bplanes) {
+    // The stages between the per-coefficient ends: the transforms, which couple
+    // every coefficient of a column.
+    void run_transforms(hipStream_t st, uint32_t nops, const uint32_t* bplanes) {
         const pm::Shape& s = host.shape;
         pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
         pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
         pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
-        pm::launch_crt(s, tables, work.rtiles, work.words, st);
     }
 
     void collect_times() {
