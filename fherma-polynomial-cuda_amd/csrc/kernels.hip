@@ -110,7 +110,7 @@ __device__ __forceinline__ int64_t combine4(const v4i& v) {
 // XOR-swizzled by row/2.  A 128-bit LDS read serves 8 lanes (128 bytes) per
 // pass; a fragment read gives those 8 lanes rows r..r+7 of one chunk, and with
 // the swizzle their 16-byte slots (row*4 + chunk') mod 8 are all distinct.  The
-// staging writes (4 chunks of rows r, r+1 per 8 lanes) are distinct as well.
+// direct-to-LDS loads fill the slots linearly (64 consecutive slots per wave).
 __device__ __forceinline__ uint32_t swz(uint32_t row, uint32_t chunk) {
     return row * kTK + (((chunk ^ (row >> 1)) & 3u) << 4);
 }
@@ -132,123 +132,104 @@ __device__ __forceinline__ void tile_coords(uint32_t num_m, uint32_t num_n, uint
     tn = local / height;
 }
 
-// B operand loaders.  Each fills one 256 x 64-byte K step per call through
-// kLoads x 16 bytes per thread and writes it into the swizzled LDS tile.
-struct TiledB {  // pre-tiled signed bytes: [k steps][256 rows][64]
-    const int8_t* base;
-    __device__ __forceinline__ void load(uint32_t ks, uint32_t tid, v4u (&rb)[kLoads]) const {
-        const v4u* src = reinterpret_cast<const v4u*>(base + size_t(ks) * kTileBytes);
-#pragma unroll
-        for (uint32_t s = 0; s < kLoads; ++s) rb[s] = src[tid + kGemmThreads * s];
-    }
-    __device__ __forceinline__ void store(int8_t* lds, uint32_t tid, const v4u (&rb)[kLoads]) const {
-#pragma unroll
-        for (uint32_t s = 0; s < kLoads; ++s) {
-            const uint32_t idx = tid + kGemmThreads * s;
-            *reinterpret_cast<v4u*>(lds + swz(idx >> 2, idx & 3)) = rb[s];
-        }
-    }
+// The K loop, for a block tile of 256 weight rows (A, the pre-tiled weight
+// panel [k steps][256][64]) x 256 coefficients (B, the pre-tiled data panel of
+// the same layout), 64 bytes of K per step, four stages in flight:
+//  * A is not staged: each wave loads its own MFMA fragments straight from
+//    global memory.  Fragment mi of step ks for lane l is the 16 bytes of row
+//    wm*64 + mi*16 + (l&15), chunk l>>4 -- 1 KB contiguous per fragment, one
+//    coalesced load.  (Waves wn = 0 and 1 load the same fragments; the second
+//    load mostly hits the CU's L1.)
+//  * B goes global -> LDS with direct-to-LDS loads (global_load_lds_dwordx4):
+//    lane l of a load fills LDS slot base + l, so each lane fetches the global
+//    16-byte chunk that the XOR-swizzled tile layout (swz) puts in that slot.
+//    No register staging and no ds_write instructions.
+//  * Loads run three steps ahead: four LDS stages (64 KB) and four register
+//    sets of A fragments rotate.  Step k waits until its own loads have landed
+//    (the 12 loads of steps k+1, k+2 may stay in flight: 4 A + 2 B per lane and
+//    step) and until this wave's LDS reads are done, then one barrier; after
+//    it, no wave still reads stage (k-1)%4, which step k+3's loads refill.
+// The barrier is a plain s_barrier with a compiler memory clobber, not
+// __syncthreads(): the latter's release fence would wait for every
+// outstanding direct-to-LDS load (vmcnt 0) and undo the prefetch.
+// Measured on the conversion and CRT shapes of N=65536, L=1747 against the
+// two-stage loop that staged both operands through registers and ds_write:
+// 2-3 % faster on the conversion GEMM, 5-10 % on the CRT GEMM.
+struct GemmStages {
+    int8_t* s[4];
 };
 
-// The K loop: A is a pre-tiled weight panel ([k steps][256][64]); LDS holds
-// two stages of (A, B) = 64 KB and one barrier ends each 64-byte K step.  Global
-// loads run two K steps ahead through two register staging sets.  Within step
-// k (LDS buffer k&1) a wave reads its fragments of step k first, then writes
-// step k+1 (fetched during step k-1) into the other buffer -- last read before
-// the previous barrier -- and issues the fetch of step k+2, then runs its MFMAs:
-// the LDS writes and the global loads overlap the MFMAs instead of sitting
-// between two barriers.  Scheduling hints place the fragment reads first and
-// then two MFMAs before each LDS write and each global load; the loop body has
-// no branches (the last fetches repeat the final step, and the last write fills
-// a buffer that is either not read again or, for an odd step count, is meant to
-// hold that final step), so the hints cover the whole step.  Wave (wm, wn)
-// computes rows wm*64 .. +64 and coefficients wn*128 .. +128 of the tile.
-template <class BLoader>
-__device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel, const BLoader& bl,
-                                              uint32_t ksteps, int8_t* lds, v4i (&acc)[kWaveM][kWaveN]) {
+__device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel, const int8_t* __restrict__ b_panel,
+                                              uint32_t ksteps, const GemmStages& st, v4i (&acc)[kWaveM][kWaveN]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
-    v4u ra0[kLoads], rb0[kLoads], ra1[kLoads], rb1[kLoads];
-    auto load = [&](uint32_t ks, v4u (&ra)[kLoads], v4u (&rb)[kLoads]) {
-        const v4u* src = reinterpret_cast<const v4u*>(a_panel + size_t(ks) * kTileBytes);
+    const v4i* ap = reinterpret_cast<const v4i*>(a_panel + (wm * kWaveRows + (lane & 15)) * kTK + (lane >> 4) * 16);
+    // B: slot s = tid + 512 j of a stage holds row s>>2, swizzled chunk s&3,
+    // i.e. the row's chunk (s&3) ^ ((row>>1)&3) in the global tile.
+    uint32_t boff[kLoads];
 #pragma unroll
-        for (uint32_t s = 0; s < kLoads; ++s) ra[s] = src[tid + kGemmThreads * s];
-        bl.load(ks, tid, rb);
+    for (uint32_t j = 0; j < kLoads; ++j) {
+        const uint32_t slot = tid + kGemmThreads * j, row = slot >> 2, chunk = (slot & 3) ^ ((row >> 1) & 3);
+        boff[j] = row * kTK + chunk * 16;
+    }
+    auto clamp = [&](uint32_t k) { return k < ksteps ? k : ksteps - 1; };  // past the end: repeat the last step
+    auto load_b = [&](uint32_t ks, int8_t* stage) {
+        const int8_t* src = b_panel + size_t(ks) * kTileBytes;
+#pragma unroll
+        for (uint32_t j = 0; j < kLoads; ++j)
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void*>(src + boff[j]),
+                                             (__attribute__((address_space(3))) void*)(stage + (wave * 64 + kGemmThreads * j) * 16),
+                                             16, 0, 0);
     };
-    auto store = [&](uint32_t buf, const v4u (&ra)[kLoads], const v4u (&rb)[kLoads]) {
-        int8_t* la = lds + buf * 2 * kTileBytes;
+    auto load_a = [&](uint32_t ks, v4i (&fa)[kWaveM]) {
 #pragma unroll
-        for (uint32_t s = 0; s < kLoads; ++s) {
-            const uint32_t idx = tid + kGemmThreads * s;
-            *reinterpret_cast<v4u*>(la + swz(idx >> 2, idx & 3)) = ra[s];
-        }
-        bl.store(la + kTileBytes, tid, rb);
+        for (uint32_t mi = 0; mi < kWaveM; ++mi) fa[mi] = ap[size_t(ks) * (kTileBytes / 16) + mi * 64];
     };
-    v4i fa[kWaveM], fb[kWaveN];
-    auto fragments = [&](uint32_t buf) {
-        const int8_t* la = lds + buf * 2 * kTileBytes;
-        const int8_t* lb = la + kTileBytes;
-#pragma unroll
-        for (uint32_t mi = 0; mi < kWaveM; ++mi)
-            fa[mi] = *reinterpret_cast<const v4i*>(la + swz(wm * kWaveRows + mi * 16 + (lane & 15), lane >> 4));
+    v4i f0[kWaveM], f1[kWaveM], f2[kWaveM], f3[kWaveM], fb[kWaveN];
+    auto step = [&](uint32_t k, const int8_t* stage, const v4i (&fa)[kWaveM], int8_t* next_stage,
+                    v4i (&next_fa)[kWaveM]) {
+        // vmcnt <= 12 (gfx9 encoding: low bits 3:0, high bits 15:14), lgkmcnt 0, expcnt untouched
+        constexpr int kWait = (12 & 15) | ((12 >> 4) << 14) | (0x7 << 4);
+        __builtin_amdgcn_s_waitcnt(kWait);
+        asm volatile("s_barrier" ::: "memory");
+        load_b(clamp(k + 3), next_stage);
+        load_a(clamp(k + 3), next_fa);
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni)
-            fb[ni] = *reinterpret_cast<const v4i*>(lb + swz(wn * kWaveCols + ni * 16 + (lane & 15), lane >> 4));
-    };
-    auto mfmas = [&]() {
+            fb[ni] = *reinterpret_cast<const v4i*>(stage + swz(wn * kWaveCols + ni * 16 + (lane & 15), lane >> 4));
 #pragma unroll
         for (uint32_t mi = 0; mi < kWaveM; ++mi)
 #pragma unroll
             for (uint32_t ni = 0; ni < kWaveN; ++ni)
                 acc[mi][ni] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[mi], fb[ni], acc[mi][ni], 0, 0, 0);
     };
-
-    load(0, ra0, rb0);
-    if (ksteps > 1) load(1, ra1, rb1);
-    store(0, ra0, rb0);
-    __syncthreads();
-    // Instruction order within one step (masks: 0x100 LDS read, 0x200 LDS write,
-    // 0x020 global load, 0x008 MFMA).  Measured at 256 x 256 tiles against the
-    // compiler's own order: 3-5% faster, and steadier from run to run.
-    auto order = [&]() {
-        constexpr int kPerOp = 2, kOps = 2 * kLoads;
-        __builtin_amdgcn_sched_group_barrier(0x100, kWaveM + kWaveN, 0);
-#pragma unroll
-        for (int i = 0; i < kOps; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, kPerOp, 0);
-            __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < kOps; ++i) {
-            __builtin_amdgcn_sched_group_barrier(0x008, kPerOp, 0);
-            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-        }
-        __builtin_amdgcn_sched_group_barrier(0x008, kWaveM * kWaveN - 2 * kPerOp * kOps, 0);
-    };
-    uint32_t ks = 0;
-    for (; ks + 2 <= ksteps; ks += 2) {
-        const uint32_t next2 = ks + 2 < ksteps ? ks + 2 : ksteps - 1;
-        const uint32_t next3 = ks + 3 < ksteps ? ks + 3 : ksteps - 1;
-        // even step ks (buffer 0): set 1 holds step ks+1 -> buffer 1; set 0 fetches ks+2
-        fragments(0);
-        store(1, ra1, rb1);
-        load(next2, ra0, rb0);
-        mfmas();
-        order();
-        __syncthreads();
-        // odd step ks+1 (buffer 1): set 0 holds step ks+2 -> buffer 0; set 1 fetches ks+3
-        fragments(1);
-        store(0, ra0, rb0);
-        load(next3, ra1, rb1);
-        mfmas();
-        order();
-        __syncthreads();
+    load_b(0, st.s[0]);
+    load_a(0, f0);
+    load_b(clamp(1), st.s[1]);
+    load_a(clamp(1), f1);
+    load_b(clamp(2), st.s[2]);
+    load_a(clamp(2), f2);
+    uint32_t k0 = 0;
+    for (; k0 + 4 <= ksteps; k0 += 4) {
+        step(k0, st.s[0], f0, st.s[3], f3);
+        step(k0 + 1, st.s[1], f1, st.s[0], f0);
+        step(k0 + 2, st.s[2], f2, st.s[1], f1);
+        step(k0 + 3, st.s[3], f3, st.s[2], f2);
     }
-    if (ks < ksteps) {  // odd step count: the last step sits in buffer 0
-        fragments(0);
-        mfmas();
-    }
+    if (k0 < ksteps) step(k0, st.s[0], f0, st.s[3], f3);
+    if (k0 + 1 < ksteps) step(k0 + 1, st.s[1], f1, st.s[0], f0);
+    if (k0 + 2 < ksteps) step(k0 + 2, st.s[2], f2, st.s[1], f1);
+    // No direct-to-LDS load may outlive the block's LDS (the loads past the
+    // last step refill stages nobody reads).
+    __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));  // vmcnt 0
 }
+
+// The four 16 KB LDS stages of gemm_mainloop, as separate arrays of the kernel
+// (so the compiler can tell a stage being filled from the one being read).
+#define PM_GEMM_STAGES(name)                                                                               \
+    __shared__ __attribute__((aligned(16))) int8_t name##0[kTileBytes], name##1[kTileBytes],                \
+        name##2[kTileBytes], name##3[kTileBytes];                                                          \
+    const GemmStages name{{name##0, name##1, name##2, name##3}}
 
 // ---------------------------------------------------------------------------
 // 0. operand packing: [N][L] u32 -> [2][n_pad/256][conv_k/64][256][64] bytes,
@@ -306,7 +287,7 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
                                                        uint32_t r, uint32_t r_pad, uint32_t conv_rows,
                                                        uint32_t ksteps, uint32_t tile0, uint32_t tiles,
                                                        uint32_t op0, uint32_t nops) {
-    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    PM_GEMM_STAGES(stages);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t ntiles = n_pad / kTile, num_m = conv_rows / kTile;
@@ -314,14 +295,14 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
     tile_coords(num_m, nops * tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles) of nops operands
     const uint32_t op = op0 + tn / tiles, tcol = tile0 + tn % tiles;
     const int8_t* a_panel = wtiles + (size_t(op) * num_m + tm) * ksteps * kTileBytes;
-    TiledB bl{packed + (size_t(op) * ntiles + tcol) * ksteps * kTileBytes};
+    const int8_t* b_panel = packed + (size_t(op) * ntiles + tcol) * ksteps * kTileBytes;
 
     v4i acc[kWaveM][kWaveN];
 #pragma unroll
     for (uint32_t mi = 0; mi < kWaveM; ++mi)
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
-    gemm_mainloop(a_panel, bl, ksteps, lds, acc);
+    gemm_mainloop(a_panel, b_panel, ksteps, stages, acc);
 
     // Lane holds rows 4*(lane/16)+j (j = digit) of each 16-row tile: one prime.
     const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tcol * kTile + wn * kWaveCols;
@@ -354,7 +335,7 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
                                                 int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
                                                 uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words,
                                                 uint32_t ws, uint32_t tile0, uint32_t tiles) {
-    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    PM_GEMM_STAGES(stages);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t num_m = crt_rows / kTile;
@@ -362,14 +343,14 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
     tile_coords(num_m, tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles)
     tn += tile0;
     const int8_t* a_panel = wtiles + size_t(tm) * ksteps * kTileBytes;
-    TiledB bl{rtiles + size_t(tn) * ksteps * kTileBytes};
+    const int8_t* b_panel = rtiles + size_t(tn) * ksteps * kTileBytes;
 
     v4i acc[kWaveM][kWaveN];
 #pragma unroll
     for (uint32_t mi = 0; mi < kWaveM; ++mi)
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
-    gemm_mainloop(a_panel, bl, ksteps, lds, acc);
+    gemm_mainloop(a_panel, b_panel, ksteps, stages, acc);
 
     const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tn * kTile + wn * kWaveCols;
     if constexpr (kByCoefficient) {
@@ -1335,7 +1316,6 @@ uint32_t column_width(const Shape& s) {
     return cw;
 }
 
-constexpr size_t kGemmLds = 4 * size_t(kTileBytes);  // two stages of (A, B)
 
 }  // namespace
 
@@ -1354,7 +1334,7 @@ void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* p
 void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
                        uint32_t tile0, uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream) {
     const uint32_t blocks = (s.conv_rows / kTile) * nops * tiles;
-    hipLaunchKernelGGL(conversion_gemm, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wconv, packed,
+    hipLaunchKernelGGL(conversion_gemm, dim3(blocks), dim3(kGemmThreads), 0, stream, t.wconv, packed,
                        t.consts, planes, s.n, s.n_pad, s.r, s.r_pad, s.conv_rows, s.conv_k / kTK, tile0, tiles,
                        op0, nops);
     check_launch("conversion_gemm");
@@ -1442,10 +1422,10 @@ void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int
     if (tiles > ntiles - tile0) tiles = ntiles - tile0;
     const uint32_t blocks = (s.crt_rows / kTile) * tiles;
     if (words_by_coefficient(s))
-        hipLaunchKernelGGL(crt_gemm<true>, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words,
+        hipLaunchKernelGGL(crt_gemm<true>, dim3(blocks), dim3(kGemmThreads), 0, stream, t.wcrt, rtiles, words,
                            s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), tile0, tiles);
     else
-        hipLaunchKernelGGL(crt_gemm<false>, dim3(blocks), dim3(kGemmThreads), kGemmLds, stream, t.wcrt, rtiles, words,
+        hipLaunchKernelGGL(crt_gemm<false>, dim3(blocks), dim3(kGemmThreads), 0, stream, t.wcrt, rtiles, words,
                            s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), tile0, tiles);
     check_launch("crt_gemm");
 }
@@ -1561,24 +1541,6 @@ size_t carve_work(const Shape& s, uint8_t* base, DeviceWork* w) {
         w->words = reinterpret_cast<int64_t*>(base + round(packed) + round(planes));
     }
     return round(packed) + round(planes) + round(words);
-}
-
-void configure_kernels() {
-    // The GEMMs use 64 KB of dynamic LDS (two stages of A and B tiles).  Plans
-    // may be created on several threads at once (server workers): set once.
-    static std::once_flag once;
-    static hipError_t status = hipSuccess;
-    std::call_once(once, [] {
-        status = hipFuncSetAttribute(reinterpret_cast<const void*>(&conversion_gemm),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
-        if (status == hipSuccess)
-            status = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm<false>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
-        if (status == hipSuccess)
-            status = hipFuncSetAttribute(reinterpret_cast<const void*>(&crt_gemm<true>),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(kGemmLds));
-    });
-    if (status != hipSuccess) throw std::runtime_error(std::string("hipFuncSetAttribute: ") + hipGetErrorString(status));
 }
 
 }  // namespace pm

@@ -47,9 +47,6 @@ inline size_t word_sum_elements(const Shape& s) {
 // Rows of the a plane the T scratch needs (the lane-per-coefficient reconstruction only).
 inline uint32_t scratch_rows(const Shape& s) { return words_by_coefficient(s) ? 0 : s.t_words; }
 
-// One-time kernel attributes (dynamic LDS limits); call before any launch.
-void configure_kernels();
-
 // Static device workspace of pm_multiply_device_scratch: kScratchBytes per GPU,
 // held from the library's first plan on that GPU until the process ends.
 constexpr size_t kScratchBytes = PM_SCRATCH_BYTES;

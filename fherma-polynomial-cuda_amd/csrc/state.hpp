@@ -336,7 +336,6 @@ struct pm_plan {
         io_b = io_a + nl;
         io_c = alloc<uint32_t>(nl);
         if (s.trivial) return;
-        pm::configure_kernels();
         pm::init_scratch();  // per-device address of pm_multiply_device_scratch's static workspace
         if (host.tables == pm::Tables::kDevice) {
             build_device_tables();
