@@ -299,6 +299,37 @@ def harness_leg(polymul, workloads, n, width, limbs, q, cases):
         shutil.rmtree(root, ignore_errors=True)
 
 
+def host_copy_rate(threads=(1, 4, 8, 16), gib=1):
+    """Host DRAM copy rate of this machine, the resource the host-vector path
+    spends besides the link: numpy copies between two `gib` GiB arrays, both
+    touched beforehand, by 1..16 threads over disjoint slices (numpy drops the
+    GIL inside a copy).  GB/s counts bytes read plus bytes written; best of 3."""
+    import threading
+
+    src = np.ones(gib << 28, dtype=np.uint32)
+    dst = np.zeros_like(src)
+    rates = {}
+    for t in threads:
+        bounds = np.linspace(0, src.size, t + 1).astype(np.int64)
+
+        def work(i):
+            np.copyto(dst[bounds[i]:bounds[i + 1]], src[bounds[i]:bounds[i + 1]])
+
+        best = None
+        for _ in range(3):
+            workers = [threading.Thread(target=work, args=(i,)) for i in range(t)]
+            t0 = time.perf_counter()
+            for w in workers:
+                w.start()
+            for w in workers:
+                w.join()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        rates["threads_%d_gbs" % t] = round(2 * src.nbytes / best / 1e9, 1)
+    del src, dst
+    return rates
+
+
 def cpu_baseline(q, a, b, plan_cls, sample_n):
     """The oracle's GMP Kronecker product (one core) on a bounded sample: the
     first sample_n coefficients of each operand as an N=sample_n product with
@@ -431,10 +462,29 @@ def summarize_traffic(fetch_csv, write_csv, dest):
     Path(dest).write_text(json.dumps(out, indent=1) + "\n")
 
 
-def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir):
+def _clocks(path, xcds=8):
+    """GPU clock during each dispatch from a `--pmc GRBM_GUI_ACTIVE --kernel-trace`
+    run: GRBM_GUI_ACTIVE counts busy cycles summed over the XCDs, so the clock
+    is that count / XCDs / the dispatch's duration.  -> {kernel: (GHz, ms)}."""
+    import csv
+    from collections import defaultdict
+
+    runs = defaultdict(list)
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != "GRBM_GUI_ACTIVE":
+                continue
+            seconds = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) * 1e-9
+            if seconds > 0:
+                runs[_short_kernel(row["Kernel_Name"])].append((float(row["Counter_Value"]) / xcds / seconds, seconds))
+    return {k: (sum(g for g, _ in v) / len(v) / 1e9, sum(t for _, t in v) / len(v) * 1e3) for k, v in runs.items()}
+
+
+def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir, clock_csv=None):
     """Markdown table of a --kernel-trace --stats run of the default product
     (5 timed steps after 1 warm-up), plus the per-dispatch SQ counters of a
-    one-step counter run when pmc_csv is given."""
+    one-step counter run when pmc_csv is given, and the clock each kernel ran at
+    when clock_csv (a GRBM_GUI_ACTIVE counter run) is given."""
     import csv
 
     steps = "--steps 5 --warmup 1 --no-cpu --no-secondary --no-batch"
@@ -465,6 +515,23 @@ def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir):
             act = avg.get("SQ_LDS_IDX_ACTIVE", 0)
             ratio = "%.2f" % (avg.get("SQ_LDS_BANK_CONFLICT", 0) / act) if act else "-"
             lines.append("| %s | " % k + " | ".join("%.4g" % avg.get(c, 0) for c in shown) + " | %s |" % ratio)
+    if clock_csv:
+        clk_dir = str(Path(clock_csv).parent)
+        lines += ["", "## Clock under load (one product, per dispatch)", "", "```",
+                  f"rocprofv3 --pmc GRBM_GUI_ACTIVE GRBM_COUNT --kernel-trace --output-format csv -d {clk_dir} -o large "
+                  f"-- python3 bench.py {PROFILE_ARGS}",
+                  "```", "",
+                  "GHz = GRBM_GUI_ACTIVE (busy cycles summed over the 8 XCDs) / 8 / dispatch duration. For the "
+                  "GEMMs, ops per cycle per SIMD = algorithmic INT8 ops / (duration x clock) / 1024 SIMDs, against "
+                  "2048 for one v_mfma_i32_16x16x64_i8 every 16 cycles (the 5.0 POP/s datasheet peak at 2.4 GHz).",
+                  "", "| kernel | GHz | ms (counter run) | INT8 ops / cycle / SIMD |", "|---|---|---|---|"]
+        gemm_ops = getattr(summarize_kernels, "gemm_ops", {})
+        for k, (ghz, ms) in sorted(_clocks(clock_csv).items(), key=lambda kv: -kv[1][1]):
+            if "copyBuffer" in k or "elementwise" in k:
+                continue
+            ops = next((v for name, v in gemm_ops.items() if name in k), None)
+            per = "%.0f" % (ops / (ms * 1e-3 * ghz * 1e9) / 1024) if ops else "-"
+            lines.append("| %s | %.2f | %.3f | %s |" % (k, ghz, ms, per))
     Path(dest).write_text("\n".join(lines) + "\n")
 
 
@@ -488,7 +555,7 @@ def main():
                         help="write profiles/traffic_large.json from two counter runs, then exit (no GPU)")
     parser.add_argument("--summarize-kernels", nargs="+", metavar="CSV",
                         help="write profiles/kernels_large.md from a kernel-stats csv and optionally an "
-                             "SQ counter csv, then exit (no GPU)")
+                             "SQ counter csv and a GRBM_GUI_ACTIVE counter csv, then exit (no GPU)")
     args = parser.parse_args()
     if args.oracle_worker:
         _oracle_worker(args.oracle_worker)
@@ -499,9 +566,13 @@ def main():
     if args.summarize_kernels:
         stats = Path(args.summarize_kernels[0])
         pmc = Path(args.summarize_kernels[1]) if len(args.summarize_kernels) > 1 else None
+        clock = Path(args.summarize_kernels[2]) if len(args.summarize_kernels) > 2 else None
+        # algorithmic ops of the two GEMMs at the default workload (3636 primes)
+        summarize_kernels.gemm_ops = {"conversion_gemm": conversion_ops(65536, 1747, 3636),
+                                      "crt_gemm": crt_ops(65536, 1747, 3636)}
         # the -d directories in the recorded commands are those the csv files came from
         summarize_kernels(stats, pmc, ROOT / "profiles" / "kernels_large.md",
-                          str(stats.parent), str(pmc.parent) if pmc else None)
+                          str(stats.parent), str(pmc.parent) if pmc else None, clock)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: one product runs on one GPU; the multi-GPU "
@@ -580,6 +651,7 @@ def main():
         e2e, c_host = time_host(plan, a, b, 3)
         result["e2e_host_ms"] = {"median": round(float(np.median(e2e)), 3), "min": round(min(e2e), 3),
                                  "runs": len(e2e)}
+        result["host_copy_rate"] = host_copy_rate()
         result["e2e_matches_device"] = bool(np.array_equal(c_host, c))
         del c_host
     plan.close()
