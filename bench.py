@@ -420,7 +420,7 @@ SQ_COUNTERS = ["SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_VALU_MFMA_BUSY_CYCLES", "
 
 
 def _short_kernel(name):
-    return name.replace("pm::(anonymous namespace)::", "").split("(")[0]
+    return name.replace("pm::", "").replace("(anonymous namespace)::", "").split("(")[0]
 
 
 def _counters(path):
@@ -480,7 +480,10 @@ def _clocks(path, xcds=8):
     return {k: (sum(g for g, _ in v) / len(v) / 1e9, sum(t for _, t in v) / len(v) * 1e3) for k, v in runs.items()}
 
 
-def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir, clock_csv=None):
+VALU_COUNTERS = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_WAVE_CYCLES"]
+
+
+def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir, clock_csv=None, valu_csv=None):
     """Markdown table of a --kernel-trace --stats run of the default product
     (5 timed steps after 1 warm-up), plus the per-dispatch SQ counters of a
     one-step counter run when pmc_csv is given, and the clock each kernel ran at
@@ -522,8 +525,9 @@ def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir, clock_csv=No
                   f"-- python3 bench.py {PROFILE_ARGS}",
                   "```", "",
                   "GHz = GRBM_GUI_ACTIVE (busy cycles summed over the 8 XCDs) / 8 / dispatch duration. For the "
-                  "GEMMs, ops per cycle per SIMD = algorithmic INT8 ops / (duration x clock) / 1024 SIMDs, against "
-                  "2048 for one v_mfma_i32_16x16x64_i8 every 16 cycles (the 5.0 POP/s datasheet peak at 2.4 GHz).",
+                  "GEMMs and the register-only MFMA calibration kernels, ops per cycle per SIMD = INT8 ops / (duration x "
+                  "clock) / 1024 SIMDs, against 2048 for one v_mfma_i32_16x16x64_i8 every 16 cycles (the 5.0 POP/s "
+                  "datasheet peak assumes 2.4 GHz). Counter collection itself slows the GEMMs somewhat.",
                   "", "| kernel | GHz | ms (counter run) | INT8 ops / cycle / SIMD |", "|---|---|---|---|"]
         gemm_ops = getattr(summarize_kernels, "gemm_ops", {})
         for k, (ghz, ms) in sorted(_clocks(clock_csv).items(), key=lambda kv: -kv[1][1]):
@@ -532,6 +536,23 @@ def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir, clock_csv=No
             ops = next((v for name, v in gemm_ops.items() if name in k), None)
             per = "%.0f" % (ops / (ms * 1e-3 * ghz * 1e9) / 1024) if ops else "-"
             lines.append("| %s | %.2f | %.3f | %s |" % (k, ghz, ms, per))
+    if valu_csv and clock_csv:
+        clocks = _clocks(clock_csv)
+        lines += ["", "## VALU issue (one product, per dispatch)", "", "```",
+                  f"rocprofv3 --pmc {' '.join(VALU_COUNTERS)} --output-format csv -d {Path(valu_csv).parent} -o large "
+                  f"-- python3 bench.py {PROFILE_ARGS}",
+                  "```", "",
+                  "VALU share = SQ_INSTS_VALU x 4 cycles (a wave64 VALU instruction on a 16-lane SIMD) / "
+                  "(1024 SIMDs x dispatch cycles at the clock above): the fraction of the SIMDs' VALU issue the "
+                  "kernel uses.", "", "| kernel | SQ_INSTS_VALU | SQ_ACTIVE_INST_VALU | VALU share |", "|---|---|---|---|"]
+        for k, d in _counters(valu_csv).items():
+            if "copyBuffer" in k or "elementwise" in k or k not in clocks:
+                continue
+            ghz, ms = clocks[k]
+            insts = sum(d["SQ_INSTS_VALU"]) / len(d["SQ_INSTS_VALU"])
+            active = sum(d["SQ_ACTIVE_INST_VALU"]) / len(d["SQ_ACTIVE_INST_VALU"])
+            share = insts * 4 / (1024 * ms * 1e-3 * ghz * 1e9)
+            lines.append("| %s | %.4g | %.4g | %.2f |" % (k, insts, active, share))
     Path(dest).write_text("\n".join(lines) + "\n")
 
 
@@ -555,7 +576,8 @@ def main():
                         help="write profiles/traffic_large.json from two counter runs, then exit (no GPU)")
     parser.add_argument("--summarize-kernels", nargs="+", metavar="CSV",
                         help="write profiles/kernels_large.md from a kernel-stats csv and optionally an "
-                             "SQ counter csv and a GRBM_GUI_ACTIVE counter csv, then exit (no GPU)")
+                             "SQ counter csv, a GRBM_GUI_ACTIVE counter csv and a VALU counter csv, then exit "
+                             "(no GPU)")
     args = parser.parse_args()
     if args.oracle_worker:
         _oracle_worker(args.oracle_worker)
@@ -567,12 +589,17 @@ def main():
         stats = Path(args.summarize_kernels[0])
         pmc = Path(args.summarize_kernels[1]) if len(args.summarize_kernels) > 1 else None
         clock = Path(args.summarize_kernels[2]) if len(args.summarize_kernels) > 2 else None
+        valu = Path(args.summarize_kernels[3]) if len(args.summarize_kernels) > 3 else None
         # algorithmic ops of the two GEMMs at the default workload (3636 primes)
+        # and of the register-only calibration launches (calibrate(): 2048 blocks x 8 waves x 2048
+        # iterations x 16 chains of 16x16x64, or x 8 chains of 32x32x32)
         summarize_kernels.gemm_ops = {"conversion_gemm": conversion_ops(65536, 1747, 3636),
-                                      "crt_gemm": crt_ops(65536, 1747, 3636)}
+                                      "crt_gemm": crt_ops(65536, 1747, 3636),
+                                      "mfma16_peak": 2048 * 8 * 2048 * 16 * 2 * 16 * 16 * 64,
+                                      "mfma32_peak": 2048 * 8 * 2048 * 8 * 2 * 32 * 32 * 32}
         # the -d directories in the recorded commands are those the csv files came from
         summarize_kernels(stats, pmc, ROOT / "profiles" / "kernels_large.md",
-                          str(stats.parent), str(pmc.parent) if pmc else None, clock)
+                          str(stats.parent), str(pmc.parent) if pmc else None, clock, valu)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: one product runs on one GPU; the multi-GPU "
@@ -580,6 +607,13 @@ def main():
               "measured path. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    # The library first: it binds the process to the HIP runtime it was built
+    # against (ROCm's libamdhip64.so.7), which torch then shares, instead of
+    # torch's bundled copy.  The two runtimes differ in how they move device ->
+    # pinned-host copies (ROCm's uses the SDMA engines; torch's copy runs them as
+    # blit kernels on the CUs, beside the CRT GEMM chunks they overlap).
+    polymul = _load("polymul")
+    polymul.load_library()
     import torch
 
     if args.calibrate_copy:
@@ -592,7 +626,6 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    polymul = _load("polymul")
     workloads = _load("workloads")
     t_gen = time.perf_counter()
     n, width, limbs, q, a, b = workloads.seeded_case("large")
@@ -642,6 +675,11 @@ def main():
             result["roofline"]["measured_peak"] = measured
             result["roofline"]["frac_of_measured_peak"] = round(achieved / measured, 4)
     result["stages_ms"] = {k: round(v, 4) for k, v in stages.items()}
+    result["hip_runtime"] = sorted(polymul.hip_runtimes())
+    # every output word of the timed product against the oracle's GMP product of
+    # the same seeded inputs (its SHA-256, computed once and committed)
+    golden = json.loads((ROOT / "tests" / "golden" / "workload_digests.json").read_text())["large"]
+    result["exact_vs_oracle_digest"] = workloads.digest(c) == golden["c"] and workloads.digest(a) == golden["a"]
     result["crt_gemm_tops"] = round(crt_ops(n, limbs, r) / (stages["crt_gemm"] * 1e-3) / 1e12, 1)
     result["setup_s"] = {"inputs": round(gen_s, 2), "plan": round(plan_s, 2)}
 

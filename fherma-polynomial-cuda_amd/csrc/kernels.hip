@@ -576,7 +576,13 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t*
                                                                  const uint2* __restrict__ tw, uint32_t ps,
                                                                  uint32_t n, uint32_t n2, uint32_t C, uint32_t r,
                                                                  uint32_t ksteps, int8_t* __restrict__ tiles) {
-    __shared__ uint32_t lds[ROWS][16][17];
+    // lds[j][column][slot]: the four 16-byte groups of a column's 16 primes are
+    // XOR-swizzled by column/2, so the transpose reads each 16-byte store as one
+    // aligned 128-bit LDS read (8 lanes of a read pass: two columns, distinct
+    // banks) and the 4-byte writes of a wave's 32-lane halves are at most 2-way
+    // conflicted (4-way unswizzled).
+    __shared__ __attribute__((aligned(16))) uint32_t lds[ROWS][16][16];
+    auto slot = [](uint32_t c, uint32_t kk) { return ((((kk >> 2) ^ (c >> 1)) & 3u) << 2) | (kk & 3u); };
     const uint32_t kk = threadIdx.x >> 4, cc = threadIdx.x & 15;
     // A block reads 64 bytes of each row: half a 128-byte line, whose other half
     // is the next column group's.  Workgroups go to the 8 XCDs round robin by
@@ -606,13 +612,13 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t*
         for (int j = 0; j < ROWS; ++j) x[j] = 0;
     }
 #pragma unroll
-    for (int j = 0; j < ROWS; ++j) lds[j][cc][kk] = x[j] ^ 0x80808080u;
+    for (int j = 0; j < ROWS; ++j) lds[j][cc][slot(cc, kk)] = x[j] ^ 0x80808080u;
     __syncthreads();
     for (uint32_t idx = threadIdx.x; idx < uint32_t(ROWS) * 64; idx += 256) {
         const uint32_t q = idx & 3, c2 = (idx >> 2) & 15, j = idx >> 6;
         const uint32_t i = j * C + c0 + c2;
         if (i < n) {
-            const v4u v{lds[j][c2][4 * q], lds[j][c2][4 * q + 1], lds[j][c2][4 * q + 2], lds[j][c2][4 * q + 3]};
+            const v4u v = *reinterpret_cast<const v4u*>(&lds[j][c2][slot(c2, 4 * q)]);
             *reinterpret_cast<v4u*>(tiles + ((size_t(i >> 8) * ksteps + kg) * 256 + (i & 255)) * 64 + q * 16) = v;
         }
     }

@@ -63,3 +63,24 @@ def test_kernel_table_lists_every_kernel(tmp_path):
     assert "| crt_gemm | 6 | 5.1000 | 5.0000 | 30.600 | 60.00 |" in text
     assert "| reconstruct | 6 | 0.7800 | 0.7600 | 4.680 | 9.20 |" in text
     assert "-d out/trace -o large" in text
+
+
+def test_clock_is_busy_cycles_per_xcd_over_the_dispatch(tmp_path):
+    """GRBM_GUI_ACTIVE sums busy cycles over the 8 XCDs: 8 x 1.8e7 cycles in
+    10 ms is 1.8 GHz; dispatches of one kernel are averaged."""
+    bench = _bench()
+    path = tmp_path / "clk.csv"
+    gemm = "pm::(anonymous namespace)::conversion_gemm(signed char const*, int)"
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Name", "Counter_Value", "Start_Timestamp",
+                                          "End_Timestamp"])
+        w.writeheader()
+        w.writerow({"Kernel_Name": gemm, "Counter_Name": "GRBM_GUI_ACTIVE", "Counter_Value": 8 * 1.8e7,
+                    "Start_Timestamp": 1000, "End_Timestamp": 1000 + 10_000_000})
+        w.writerow({"Kernel_Name": gemm, "Counter_Name": "GRBM_COUNT", "Counter_Value": 1.0,
+                    "Start_Timestamp": 1000, "End_Timestamp": 1000 + 10_000_000})
+        w.writerow({"Kernel_Name": gemm, "Counter_Name": "GRBM_GUI_ACTIVE", "Counter_Value": 8 * 2.0e7,
+                    "Start_Timestamp": 0, "End_Timestamp": 10_000_000})
+    clocks = bench._clocks(path)
+    ghz, ms = clocks["conversion_gemm"]
+    assert abs(ghz - 1.9) < 1e-9 and abs(ms - 10.0) < 1e-9
