@@ -1078,23 +1078,36 @@ __global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restri
         qe = uint64_t(((unsigned __int128)uint64_t(ts) * args.mu) >> args.k_shift);
     }
 
+    // A round's inputs (lane j of words j0 .. j0+63): D_j + corr_t_j + K*negp_j and
+    // the three words of q that word j of Qe*q needs.  Every load is
+    // unconditional, its index clamped into its table and the value selected
+    // afterwards, so a round carries no divergent branches, and the loads of a
+    // round are issued two rounds before it is normalized.  K < 2^32 (it is
+    // below the number of primes), so K*negp_j is one 32 x 32-bit product.
+    struct Round {  // the raw loads; masked and combined only when the round is used
+        int64_t d;
+        uint32_t ct, np, qj, qm1, qm2;
+    };
+    const uint32_t k32 = uint32_t(kq);
+    auto fetch = [&](uint32_t j0) {
+        const uint32_t j = j0 + lane;
+        const uint32_t jq = j < L ? j : L - 1, jq1 = j - 1 < L ? j - 1 : 0, jq2 = j - 2 < L ? j - 2 : 0;
+        return Round{d[j <= L ? j : L], corr_t[j < tw ? j : tw - 1], neg_p[jq], q[jq], q[jq1], q[jq2]};
+    };
     for (int pass = 0; pass < 6; ++pass) {
         const uint64_t q0 = uint32_t(qe), q1 = uint32_t(qe >> 32);
         int64_t carry = 0;
         int cmp = 0;  // sign of R - q over the words so far (decided by the highest differing word)
-        for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
+        auto round = [&](const Round& in, uint32_t j0) {
             const uint32_t j = j0 + lane;
-            int64_t x = 0;
-            uint32_t qj = 0;
-            if (j < tw) {
-                qj = j < L ? q[j] : 0;
-                const uint32_t qm1 = j >= 1 && j - 1 < L ? q[j - 1] : 0;
-                const uint32_t qm2 = j >= 2 && j - 2 < L ? q[j - 2] : 0;
-                // word j of Qe*q: lo(q0 q_j) + hi(q0 q_{j-1}) + lo(q1 q_{j-1}) + hi(q1 q_{j-2})
-                const uint64_t pa = q0 * qj, pb = q0 * qm1, pc = q1 * qm1, pd = q1 * qm2;
-                const int64_t sub = int64_t(uint32_t(pa)) + int64_t(pb >> 32) + int64_t(uint32_t(pc)) + int64_t(pd >> 32);
-                x = (j <= L ? d[j] : 0) + int64_t(corr_t[j]) + (j < L ? kq * int64_t(neg_p[j]) : 0) - sub;
-            }
+            const uint32_t qj = j < L ? in.qj : 0;
+            const uint32_t qm1 = j >= 1 && j - 1 < L ? in.qm1 : 0;
+            const uint32_t qm2 = j >= 2 && j - 2 < L ? in.qm2 : 0;
+            // word j of Qe*q: lo(q0 q_j) + hi(q0 q_{j-1}) + lo(q1 q_{j-1}) + hi(q1 q_{j-2})
+            const uint64_t pa = q0 * qj, pb = q0 * qm1, pc = q1 * qm1, pd = q1 * qm2;
+            const int64_t sub = int64_t(uint32_t(pa)) + int64_t(pb >> 32) + int64_t(uint32_t(pc)) + int64_t(pd >> 32);
+            const int64_t kr = j < L ? int64_t(uint64_t(k32) * in.np) : 0;
+            const int64_t x = j < tw ? (j <= L ? in.d : 0) + int64_t(in.ct) + kr - sub : 0;
             const uint32_t w = normalize_round(x, carry, lane);
             if (j < L) out[j] = w;
             const uint64_t differ = __ballot(w != qj);
@@ -1104,6 +1117,19 @@ __global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restri
                           ? 1
                           : -1;
             }
+        };
+        // Three register sets in rotation (no copies between them, which would
+        // wait for the loads): each set is refilled right after its round.
+        Round r0 = fetch(0), r1 = fetch(64), r2;
+        for (uint32_t j0 = 0; j0 < tw; j0 += 192) {
+            r2 = fetch(j0 + 128);
+            round(r0, j0);
+            if (j0 + 64 >= tw) break;
+            r0 = fetch(j0 + 192);
+            round(r1, j0 + 64);
+            if (j0 + 128 >= tw) break;
+            r1 = fetch(j0 + 256);
+            round(r2, j0 + 128);
         }
         if (carry < 0) {
             --qe;  // R < 0
