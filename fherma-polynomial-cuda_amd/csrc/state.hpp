@@ -242,10 +242,22 @@ struct pm_plan {
                     const pm::CopyPool::Segment segs[2] = {{lane->pin_in + base, a + r0 * L, rows * row_bytes},
                                                            {lane->pin_in + base + rows * L, b ? b + r0 * L : nullptr,
                                                             rows * row_bytes}};
-                    pool->copy(segs, nops);
-                    hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
-                                             hipMemcpyHostToDevice, copy_in),
-                              "H2D inputs");
+                    if (ch == 0 && nops == 2) {
+                        // Nothing overlaps the first chunk's host copy, so it goes in
+                        // two halves: a's rows, whose H2D then runs while b's rows
+                        // are copied, then b's rows.
+                        for (uint32_t op = 0; op < 2; ++op) {
+                            pool->copy(&segs[op], 1);
+                            hip_check(hipMemcpyAsync(io_a + base + op * rows * L, segs[op].dst, rows * row_bytes,
+                                                     hipMemcpyHostToDevice, copy_in),
+                                      "H2D inputs");
+                        }
+                    } else {
+                        pool->copy(segs, nops);
+                        hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
+                                                 hipMemcpyHostToDevice, copy_in),
+                                  "H2D inputs");
+                    }
                 }
                 hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
