@@ -144,10 +144,10 @@ pm_status pm_operand_create(pm_plan* plan, const uint32_t* b, pm_operand** out) 
         op->planes = static_cast<uint32_t*>(mem);
         hipStream_t st = plan->lane->stream;
         const size_t bytes = size_t(s.n) * s.limbs * sizeof(uint32_t);
-        const uint32_t ntiles = s.n_pad / 256;
+        const pm::TileSet all = pm::all_tiles(s);
         hip_check(hipMemcpyAsync(plan->io_b, b, bytes, hipMemcpyHostToDevice, st), "copy b to device");
-        pm::launch_pack(s, nullptr, plan->io_b, plan->work.packed, 0, ntiles, 1, 1, st);
-        pm::launch_conversion(s, plan->tables, plan->work.packed, plan->work.planes, 0, ntiles, 1, 1, st);
+        pm::launch_pack(s, nullptr, plan->io_b, plan->work.packed, all, 1, 1, st);
+        pm::launch_conversion(s, plan->tables, plan->work.packed, plan->work.planes, all, 1, 1, st);
         pm::launch_ntt_forward_columns(s, plan->tables, plan->work.planes, 1, 1, st);
         hip_check(hipMemcpyAsync(op->planes, plan->b_planes(), plane_words * sizeof(uint32_t),
                                  hipMemcpyDeviceToDevice, st),

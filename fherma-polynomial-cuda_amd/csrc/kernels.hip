@@ -239,23 +239,25 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
 //    each 128-byte line of the input is consumed by one wave (a block holds 4
 //    consecutive rows, so only lines at block edges are shared).
 // ---------------------------------------------------------------------------
-// Processes coefficient tiles [tile0, tile0 + tiles) of operands op0 .. op0+nops-1.
+// Packs the tiles of `ts` of operands op0 .. op0+nops-1.  The k-th tile's rows
+// are read from rows k*256 .. k*256+255 of a (or b): the caller passes the
+// operands' rows in the set's order (all of [N][L] for the set of every tile).
 __global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict__ a,
                                                      const uint32_t* __restrict__ b,
                                                      uint32_t* __restrict__ packed, uint32_t n, uint32_t limbs,
-                                                     uint32_t ksteps, uint32_t ntiles, uint32_t tile0,
-                                                     uint32_t tiles, uint32_t op0, uint32_t nops) {
+                                                     uint32_t ksteps, uint32_t ntiles, TileSet ts, uint32_t op0,
+                                                     uint32_t nops) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t per_op = tiles * kTile;
+    const uint32_t per_op = ts.count * kTile;
     const uint32_t gr = blockIdx.x * 4 + (threadIdx.x >> 6);  // row over all operands of the launch
     if (gr >= nops * per_op) return;
     const uint32_t op = op0 + gr / per_op, rr = gr % per_op;
-    const uint32_t tn = tile0 + rr / kTile, row = rr % kTile, coef = tn * kTile + row;
+    const uint32_t tn = tile_at(ts, rr / kTile), row = rr % kTile, coef = tn * kTile + row;
     const bool live = coef < n;
-    // Rows past n read (and discard) row 0 of their operand, so every load is
+    // Rows past n read (and discard) the set's first row, so every load is
     // unconditional: kPackLoads loads per lane are in flight before their
     // stores, instead of one load, a wait and a store at a time.
-    const uint32_t* src = (op ? b : a) + size_t(live ? coef : 0) * limbs;
+    const uint32_t* src = (op ? b : a) + size_t(live ? rr : 0) * limbs;
     uint32_t* dst = packed + (size_t(op) * ntiles + tn) * ksteps * (kTileBytes / 4) + row * (kTK / 4);
     const uint32_t words = ksteps * (kTK / 4);
     constexpr uint32_t kPackLoads = 8;
@@ -285,15 +287,14 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
                                                        const PrimeConst* __restrict__ consts,
                                                        uint32_t* __restrict__ planes, uint32_t n, uint32_t n_pad,
                                                        uint32_t r, uint32_t r_pad, uint32_t conv_rows,
-                                                       uint32_t ksteps, uint32_t tile0, uint32_t tiles,
-                                                       uint32_t op0, uint32_t nops) {
+                                                       uint32_t ksteps, TileSet ts, uint32_t op0, uint32_t nops) {
     PM_GEMM_STAGES(stages);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t ntiles = n_pad / kTile, num_m = conv_rows / kTile;
     uint32_t tm, tn;
-    tile_coords(num_m, nops * tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles) of nops operands
-    const uint32_t op = op0 + tn / tiles, tcol = tile0 + tn % tiles;
+    tile_coords(num_m, nops * ts.count, tm, tn);  // the set's coefficient tiles of nops operands
+    const uint32_t op = op0 + tn / ts.count, tcol = tile_at(ts, tn % ts.count);
     const int8_t* a_panel = wtiles + (size_t(op) * num_m + tm) * ksteps * kTileBytes;
     const int8_t* b_panel = packed + (size_t(op) * ntiles + tcol) * ksteps * kTileBytes;
 
@@ -334,14 +335,14 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
                                                 const int8_t* __restrict__ rtiles,
                                                 int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
                                                 uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words,
-                                                uint32_t ws, uint32_t tile0, uint32_t tiles) {
+                                                uint32_t ws, TileSet ts) {
     PM_GEMM_STAGES(stages);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t num_m = crt_rows / kTile;
     uint32_t tm, tn;
-    tile_coords(num_m, tiles, tm, tn);  // coefficient tiles [tile0, tile0 + tiles)
-    tn += tile0;
+    tile_coords(num_m, ts.count, tm, tn);  // the set's coefficient tiles
+    tn = tile_at(ts, tn);
     const int8_t* a_panel = wtiles + size_t(tm) * ksteps * kTileBytes;
     const int8_t* b_panel = rtiles + size_t(tn) * ksteps * kTileBytes;
 
@@ -541,12 +542,14 @@ __device__ __forceinline__ void gs_butterfly(uint32_t& a, uint32_t& b, uint2 w, 
 
 // Forward row-spanning stages: one thread per (operand, prime, column) holding
 // the column's ROWS values; the stage twiddles are uniform over the block.
+// Columns col0 + [0, 256 * gridDim.x).
 template <int ROWS>
 __global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restrict__ planes,
                                                                const PrimeConst* __restrict__ consts,
                                                                const uint2* __restrict__ tw, uint32_t ps,
-                                                               uint32_t n2, uint32_t C, uint32_t r, uint32_t op0) {
-    const uint32_t col = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, op = op0 + blockIdx.z;
+                                                               uint32_t n2, uint32_t C, uint32_t r, uint32_t op0,
+                                                               uint32_t col0) {
+    const uint32_t col = col0 + blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, op = op0 + blockIdx.z;
     const uint32_t p = consts[k].p;
     uint32_t* data = planes + (size_t(op) * r + k) * ps + col;
     const uint2* w = tw + size_t(k) * n2;
@@ -569,13 +572,15 @@ __global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restr
 // residues v_{k,i} are written, XOR 0x80808080, into the CRT GEMM's operand
 // tiles [n_pad/256][r_pad/16][256 coefficients][16 primes x 4 bytes] through
 // an LDS transpose, so each coefficient's 64-byte K chunk is one store.
-// ROWS = 1 is a pure transpose (no stages).
+// ROWS = 1 is a pure transpose (no stages).  Column groups g0 + [0, gridDim.x)
+// of 16 columns each.
 template <int ROWS>
 __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t* __restrict__ plane,
                                                                  const PrimeConst* __restrict__ consts,
                                                                  const uint2* __restrict__ tw, uint32_t ps,
                                                                  uint32_t n, uint32_t n2, uint32_t C, uint32_t r,
-                                                                 uint32_t ksteps, int8_t* __restrict__ tiles) {
+                                                                 uint32_t ksteps, int8_t* __restrict__ tiles,
+                                                                 uint32_t g0) {
     // lds[j][column][slot]: the four 16-byte groups of a column's 16 primes are
     // XOR-swizzled by column/2, so the transpose reads each 16-byte store as one
     // aligned 128-bit LDS read (8 lanes of a read pass: two columns, distinct
@@ -591,7 +596,7 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t*
     // read of each line hits that XCD's L2 instead of memory.
     const uint32_t bx = blockIdx.x;
     const uint32_t cx = (gridDim.x & 15u) == 0 ? (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u) : bx;
-    const uint32_t c0 = cx * 16, kg = blockIdx.y, k = kg * 16 + kk;
+    const uint32_t c0 = (g0 + cx) * 16, kg = blockIdx.y, k = kg * 16 + kk;
     uint32_t x[ROWS];
     if (k < r) {
         const uint32_t p = consts[k].p;
@@ -802,10 +807,13 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
 }
 
 // ---------------------------------------------------------------------------
-// 6. reconstruction: one lane per coefficient, words in [j][n_pad] layout.
+// 6. reconstruction: one lane per coefficient, for q of up to 62 limbs (T of at
+// most 64 words), words in [j][n_pad] layout.  Output row k*256 + u is
+// coefficient tile_at(ts, k)*256 + u.
 // ---------------------------------------------------------------------------
 struct ReconstructArgs {
-    uint32_t n, n_pad, limbs, t_words, i_begin, i_end;
+    uint32_t n, n_pad, limbs, t_words;
+    TileSet ts;
     uint32_t s_shift, k_shift;
     uint64_t mu;
 };
@@ -818,9 +826,11 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
                                                   const uint32_t* __restrict__ corr_e,
                                                   ReconstructArgs args) {
     __shared__ uint32_t tile[64][65];
-    const uint32_t lane = threadIdx.x, i0 = args.i_begin + blockIdx.x * 64, i = i0 + lane;
-    const uint32_t n = args.n, ps = args.n_pad, L = args.limbs, tw = args.t_words;
-    const bool valid = i < args.i_end;
+    // Output rows k0 .. k0+63 lie in one tile of the set: coefficients i0 .. i0+63.
+    const uint32_t lane = threadIdx.x, k0 = blockIdx.x * 64;
+    const uint32_t i0 = tile_at(args.ts, k0 / 256) * 256 + k0 % 256, i = i0 + lane;
+    const uint32_t ps = args.n_pad, L = args.limbs, tw = args.t_words;
+    const bool valid = i < args.n;
     const uint32_t ic = valid ? i : 0;  // clamp reads of idle lanes
 
     // K = floor((E + 2^63) / 2^64), E = sum_j 2^32j words[L+1+j] + corr_e
@@ -879,15 +889,18 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     }
     const unsigned __int128 topv = (unsigned __int128)top[0] | ((unsigned __int128)top[1] << 32) |
                                    ((unsigned __int128)top[2] << 64);
-    const uint64_t ts = uint64_t(topv >> sb);
-    const uint64_t qe = uint64_t(((unsigned __int128)ts * args.mu) >> args.k_shift);
+    const uint64_t t_scaled = uint64_t(topv >> sb);
+    const uint64_t qe = uint64_t(((unsigned __int128)t_scaled * args.mu) >> args.k_shift);
 
-    // Pass 2: rem = T - qe*q (in [0, 2q)), emitted through an LDS transpose of 64
-    // words at a time; a second borrow chain tracks rem - q.  T is read in
-    // halves of 32 words, the next half in flight while the current is reduced.
-    uint64_t pc = 0;
-    uint32_t bo = 0, bo2 = 0;
+    // Pass 2: rem = T - qe*q (in [0, 2q)) into an LDS row per lane, in two
+    // halves of 32 words (the second in flight while the first is reduced); a
+    // second borrow chain tracks rem - q, and when rem >= q the row is reduced
+    // once more in LDS.  The block's rows are then stored as one contiguous run
+    // of words: c is written once and never read back, so it may be pinned host
+    // memory.
     {
+        uint64_t pc = 0;
+        uint32_t bo = 0, bo2 = 0;
         constexpr uint32_t kH = 32;
         uint32_t ta[kH], tb[kH];
         auto fetch = [&](uint32_t (&t)[kH], uint32_t j0) {
@@ -911,67 +924,31 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
                 tile[lane][col0 + u] = rj;
             }
         };
-        auto flush = [&](uint32_t j0) {  // row-wise coalesced store of c[i0+row][j0 .. j0+64) for j < L
-            __syncthreads();
-            const uint32_t jend = j0 + 64 < tw ? j0 + 64 : tw;
-            for (uint32_t row = 0; row < 64; ++row) {
-                const uint32_t gi = i0 + row, j = j0 + lane;
-                if (gi < args.i_end && j < jend && j < L) c[size_t(gi) * L + j] = tile[row][lane];
-            }
-            __syncthreads();
-        };
         fetch(ta, 0);
-        if (tw <= 64) {
-            // One group holds the whole remainder: finish it in LDS (subtract q
-            // when rem >= q) and store the block's rows as one contiguous run of
-            // 256-byte-aligned words.  c is written once and never read back, so
-            // it may be pinned host memory.
-            if (kH < tw) fetch(tb, kH);
-            reduce_half(ta, 0, 0);
-            if (kH < tw) reduce_half(tb, kH, kH);
-            if (bo2 == 0) {
-                uint32_t borrow = 0;
-                for (uint32_t j = 0; j < L; ++j) {
-                    const uint64_t d = uint64_t(tile[lane][j]) - q[j] - borrow;
-                    tile[lane][j] = uint32_t(d);
-                    borrow = uint32_t(d >> 63);
-                }
+        if (kH < tw) fetch(tb, kH);
+        reduce_half(ta, 0, 0);
+        if (kH < tw) reduce_half(tb, kH, kH);
+        if (bo2 == 0) {
+            uint32_t borrow = 0;
+            for (uint32_t j = 0; j < L; ++j) {
+                const uint64_t d = uint64_t(tile[lane][j]) - q[j] - borrow;
+                tile[lane][j] = uint32_t(d);
+                borrow = uint32_t(d >> 63);
             }
-            __syncthreads();
-            const uint32_t rows = args.i_end - i0 < 64 ? args.i_end - i0 : 64;
-            uint32_t* out = c + size_t(i0) * L;
-            const uint32_t dq = 64 / L, dr = 64 % L;  // (row, col) of word k, stepped by 64 words
-            uint32_t row = lane / L, col = lane % L;
-            for (uint32_t k = lane; k < rows * L; k += 64) {
-                out[k] = tile[row][col];
-                row += dq;
-                col += dr;
-                if (col >= L) {
-                    col -= L;
-                    ++row;
-                }
-            }
-            return;
         }
-        for (uint32_t j0 = 0; j0 < tw; j0 += 64) {
-            if (j0 + kH < tw) fetch(tb, j0 + kH);
-            reduce_half(ta, j0, 0);
-            if (j0 + kH < tw) {
-                if (j0 + 64 < tw) fetch(ta, j0 + 64);
-                reduce_half(tb, j0 + kH, kH);
+        __syncthreads();
+        const uint32_t rows = i0 < args.n ? (args.n - i0 < 64 ? args.n - i0 : 64) : 0;
+        uint32_t* out = c + size_t(k0) * L;
+        const uint32_t dq = 64 / L, dr = 64 % L;  // (row, col) of word k, stepped by 64 words
+        uint32_t row = lane / L, col = lane % L;
+        for (uint32_t k = lane; k < rows * L; k += 64) {
+            out[k] = tile[row][col];
+            row += dq;
+            col += dr;
+            if (col >= L) {
+                col -= L;
+                ++row;
             }
-            flush(j0);
-        }
-    }
-    // rem >= q (the estimate was one short): subtract q once more, in place.
-    __syncthreads();
-    if (valid && bo2 == 0) {
-        uint32_t borrow = 0;
-        uint32_t* ci = c + size_t(i) * L;
-        for (uint32_t j = 0; j < L; ++j) {
-            const uint64_t d = uint64_t(ci[j]) - q[j] - borrow;
-            ci[j] = uint32_t(d);
-            borrow = uint32_t(d >> 63);
         }
     }
 }
@@ -995,7 +972,8 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
 // moved by one and the pass repeats, rewriting c.
 // ---------------------------------------------------------------------------
 struct WaveReconstructArgs {
-    uint32_t n, ws, limbs, t_words, i_begin, i_end;
+    uint32_t n, ws, limbs, t_words;
+    TileSet ts;  // output row k*256 + u is coefficient tile_at(ts, k)*256 + u
     uint32_t top_m, top_e, k_shift;
     uint64_t mu;
 };
@@ -1044,11 +1022,13 @@ __global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restri
                                                          const uint32_t* __restrict__ corr_e,
                                                          WaveReconstructArgs args) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t i = args.i_begin + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= args.i_end) return;  // whole waves
+    const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);  // output row
+    if (k >= args.ts.count * 256) return;  // whole waves
+    const uint32_t i = tile_at(args.ts, k / 256) * 256 + k % 256;
+    if (i >= args.n) return;
     const uint32_t L = args.limbs, tw = args.t_words;
     const int64_t* d = words + size_t(i) * args.ws;
-    uint32_t* out = c + size_t(i) * L;
+    uint32_t* out = c + size_t(k) * L;
 
     // K = floor((E + 2^63) / 2^64)
     uint32_t ew[4];
@@ -1348,41 +1328,61 @@ uint32_t column_width(const Shape& s) {
     return cw;
 }
 
+// Every tile of `ts` must be one of the plan's: a set past them would have the
+// kernels read and write past the working buffers.
+void check_tiles(const Shape& s, const TileSet& ts, const char* what) {
+    const uint64_t ntiles = s.n_pad / kTile;
+    const bool ok = ts.count != 0 && ts.w != 0 && ts.count % ts.w == 0 &&
+                    uint64_t(ts.t0) + uint64_t(ts.count / ts.w - 1) * ts.stride + (ts.w - 1) < ntiles;
+    if (!ok) throw std::invalid_argument(std::string(what) + ": coefficient tiles outside the plan's");
+}
+
+// Strips [strip_begin, strip_end) of 256 columns; (0, ~0u) is every column.
+bool whole_columns(const Shape& s, uint32_t strip_begin, uint32_t strip_end, const char* what) {
+    if (strip_begin == 0 && strip_end == ~0u) return true;
+    if (s.row_width % 256 != 0 || s.rows < 2 || s.rows > 64 || strip_begin >= strip_end ||
+        strip_end > s.row_width / 256)
+        throw std::invalid_argument(std::string(what) + ": column strips outside the transform's columns");
+    return false;
+}
 
 }  // namespace
 
 size_t packed_operand_bytes(const Shape& s) { return size_t(2) * s.n_pad * s.conv_k; }
 
-void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, uint32_t tile0,
-                 uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream) {
+void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, const TileSet& ts,
+                 uint32_t op0, uint32_t nops, hipStream_t stream) {
+    check_tiles(s, ts, "pack_operands");
     const uint32_t ksteps = s.conv_k / kTK, ntiles = s.n_pad / kTile;
-    const size_t rows = size_t(nops) * tiles * kTile;  // one wave per row, 4 rows per block
+    const size_t rows = size_t(nops) * ts.count * kTile;  // one wave per row, 4 rows per block
     dim3 grid(uint32_t((rows + 3) / 4));
     hipLaunchKernelGGL(pack_operands, grid, dim3(256), 0, stream, a, b, reinterpret_cast<uint32_t*>(packed), s.n,
-                       s.limbs, ksteps, ntiles, tile0, tiles, op0, nops);
+                       s.limbs, ksteps, ntiles, ts, op0, nops);
     check_launch("pack_operands");
 }
 
 void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
-                       uint32_t tile0, uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream) {
-    const uint32_t blocks = (s.conv_rows / kTile) * nops * tiles;
+                       const TileSet& ts, uint32_t op0, uint32_t nops, hipStream_t stream) {
+    check_tiles(s, ts, "conversion_gemm");
+    const uint32_t blocks = (s.conv_rows / kTile) * nops * ts.count;
     hipLaunchKernelGGL(conversion_gemm, dim3(blocks), dim3(kGemmThreads), 0, stream, t.wconv, packed,
-                       t.consts, planes, s.n, s.n_pad, s.r, s.r_pad, s.conv_rows, s.conv_k / kTK, tile0, tiles,
-                       op0, nops);
+                       t.consts, planes, s.n, s.n_pad, s.r, s.r_pad, s.conv_rows, s.conv_k / kTK, ts, op0, nops);
     check_launch("conversion_gemm");
 }
 
 void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, uint32_t op0,
-                                uint32_t nops, hipStream_t stream) {
+                                uint32_t nops, hipStream_t stream, uint32_t strip_begin, uint32_t strip_end) {
+    const bool whole = whole_columns(s, strip_begin, strip_end, "ntt_columns_forward");
     if (s.rows < 2) return;
     const uint32_t ps = s.n_pad, r = s.r_pad, C = s.row_width;
     if (C % 256 == 0 && s.rows <= 64) {
-        dim3 grid(C / 256, s.r, nops);
+        const uint32_t s0 = whole ? 0 : strip_begin, s1 = whole ? C / 256 : strip_end;
+        dim3 grid(s1 - s0, s.r, nops);
         switch (s.rows) {
 #define PM_FWD(R)                                                                                                \
     case R:                                                                                                      \
         hipLaunchKernelGGL(ntt_columns_forward_reg<R>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_fwd, ps, \
-                           s.n2, C, r, op0);                                                                     \
+                           s.n2, C, r, op0, s0 * 256);                                                           \
         break;
             PM_FWD(2) PM_FWD(4) PM_FWD(8) PM_FWD(16) PM_FWD(32) PM_FWD(64)
 #undef PM_FWD
@@ -1418,15 +1418,17 @@ void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* pl
 }
 
 void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
-                                hipStream_t stream) {
+                                hipStream_t stream, uint32_t strip_begin, uint32_t strip_end) {
+    const bool whole = whole_columns(s, strip_begin, strip_end, "ntt_columns_inverse");
     const uint32_t ps = s.n_pad, ksteps = s.r_pad / 16;
     if (s.row_width % 16 == 0 && s.rows <= 64) {
-        dim3 grid(s.row_width / 16, s.r_pad / 16);
+        const uint32_t g0 = whole ? 0 : strip_begin * 16, g1 = whole ? s.row_width / 16 : strip_end * 16;
+        dim3 grid(g1 - g0, s.r_pad / 16);
         switch (s.rows) {
 #define PM_INV(R)                                                                                              \
     case R:                                                                                                    \
         hipLaunchKernelGGL(ntt_columns_inverse_tiles<R>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_inv, \
-                           ps, s.n, s.n2, s.row_width, s.r, ksteps, rtiles);                                   \
+                           ps, s.n, s.n2, s.row_width, s.r, ksteps, rtiles, g0);                               \
         break;
             PM_INV(1) PM_INV(2) PM_INV(4) PM_INV(8) PM_INV(16) PM_INV(32) PM_INV(64)
 #undef PM_INV
@@ -1442,44 +1444,41 @@ void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t*
         }
         dim3 grid((s.n + 15) / 16, s.r_pad / 16);
         hipLaunchKernelGGL(ntt_columns_inverse_tiles<1>, grid, dim3(256), 0, stream, planes, t.consts, t.tw_inv, ps,
-                           s.n, s.n2, s.n, s.r, ksteps, rtiles);
+                           s.n, s.n2, s.n, s.r, ksteps, rtiles, 0u);
     }
     check_launch("ntt_columns_inverse_tiles");
 }
 
-void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream,
-                uint32_t tile0, uint32_t tiles) {
-    const uint32_t ntiles = s.n_pad / kTile;
-    if (tile0 >= ntiles || tiles == 0) return;
-    if (tiles > ntiles - tile0) tiles = ntiles - tile0;
-    const uint32_t blocks = (s.crt_rows / kTile) * tiles;
+void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, const TileSet& ts,
+                hipStream_t stream) {
+    check_tiles(s, ts, "crt_gemm");
+    const uint32_t blocks = (s.crt_rows / kTile) * ts.count;
     if (words_by_coefficient(s))
         hipLaunchKernelGGL(crt_gemm<true>, dim3(blocks), dim3(kGemmThreads), 0, stream, t.wcrt, rtiles, words,
-                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), tile0, tiles);
+                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), ts);
     else
         hipLaunchKernelGGL(crt_gemm<false>, dim3(blocks), dim3(kGemmThreads), 0, stream, t.wcrt, rtiles, words,
-                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), tile0, tiles);
+                           s.n, s.n_pad, s.crt_rows, s.crt_k / kTK, s.crt_words, word_stride(s), ts);
     check_launch("crt_gemm");
 }
 
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
-                        uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream) {
-    if (!count) return;
-    const uint32_t i_end = i_begin + count < s.n ? i_begin + count : s.n;
+                        uint32_t* c, const TileSet& ts, hipStream_t stream) {
+    check_tiles(s, ts, "reconstruct");
     if (words_by_coefficient(s)) {
         // Barrett window: the four words of x from m up (m = floor(s/32) - 1, or 0 below s = 64)
         const uint32_t top_m = s.s_shift >= 64 ? s.s_shift / 32 - 1 : 0;
-        WaveReconstructArgs args{s.n, word_stride(s), s.limbs, s.t_words, i_begin, i_end,
+        WaveReconstructArgs args{s.n, word_stride(s), s.limbs, s.t_words, ts,
                                  top_m, s.s_shift - 32 * top_m, s.k_shift, s.mu};
-        hipLaunchKernelGGL(reconstruct_waves, dim3((i_end - i_begin + 3) / 4), dim3(256), 0, stream, words, c, t.q,
-                           t.neg_p, t.corr_t, t.corr_e, args);
+        hipLaunchKernelGGL(reconstruct_waves, dim3(ts.count * 64), dim3(256), 0, stream, words, c, t.q, t.neg_p,
+                           t.corr_t, t.corr_e, args);
         check_launch("reconstruct_waves");
         return;
     }
-    ReconstructArgs args{s.n, s.n_pad, s.limbs, s.t_words, i_begin, i_end, s.s_shift, s.k_shift, s.mu};
-    dim3 grid((count + 63) / 64);
-    hipLaunchKernelGGL(reconstruct, grid, dim3(64), 0, stream, words, t_scratch, c, t.q, t.neg_p, t.corr_t,
-                       t.corr_e, args);
+    if (s.t_words > 64) throw std::logic_error("reconstruct: the lane-per-coefficient kernel takes T of 64 words at most");
+    ReconstructArgs args{s.n, s.n_pad, s.limbs, s.t_words, ts, s.s_shift, s.k_shift, s.mu};
+    hipLaunchKernelGGL(reconstruct, dim3(ts.count * 4), dim3(64), 0, stream, words, t_scratch, c, t.q, t.neg_p,
+                       t.corr_t, t.corr_e, args);
     check_launch("reconstruct");
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <string>
 
+#include "chunks.hpp"
 #include "plan.hpp"
 #include "polymul.h"
 
@@ -61,32 +62,44 @@ size_t carve_work(const Shape& s, uint8_t* base, DeviceWork* w);
 size_t work_bytes(const Shape& s);
 size_t packed_operand_bytes(const Shape& s);
 
-// Stage launches, all on `stream`.
-// Coefficient tiles [tile0, tile0 + tiles) (256 coefficients each) of operands
-// op0 .. op0+nops-1 (0 = a, 1 = b; operand b uses its own weights and offsets).
-void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, uint32_t tile0,
-                 uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream);
+// Stage launches, all on `stream`.  A TileSet names the 256-coefficient tiles a
+// launch covers (every tile of it must be one of the plan's, or the launch
+// throws std::invalid_argument); all_tiles(s) is every tile.
+inline TileSet all_tiles(const Shape& s) { return tile_range(0, s.n_pad / 256); }
+// Tiles of `ts` of operands op0 .. op0+nops-1 (0 = a, 1 = b; operand b uses its
+// own weights and offsets).  pack reads the k-th tile's rows from rows
+// k*256 .. k*256+255 of a and b (for all_tiles: [N][L] as given).
+void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* packed, const TileSet& ts,
+                 uint32_t op0, uint32_t nops, hipStream_t stream);
 void launch_conversion(const Shape& s, const DeviceTables& t, const int8_t* packed, uint32_t* planes,
-                       uint32_t tile0, uint32_t tiles, uint32_t op0, uint32_t nops, hipStream_t stream);
+                       const TileSet& ts, uint32_t op0, uint32_t nops, hipStream_t stream);
+// Forward row-spanning (column) stages of column strips [strip_begin,
+// strip_end) of 256 columns each, or of every column with the defaults; a strip
+// range needs the register-resident kernel (row_width a multiple of 256, 2 to
+// 64 rows).
 void launch_ntt_forward_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, uint32_t op0,
-                                uint32_t nops, hipStream_t stream);
+                                uint32_t nops, hipStream_t stream, uint32_t strip_begin = 0,
+                                uint32_t strip_end = ~0u);
 // Reads a's planes from `planes` and b's from `bplanes` ([r_pad][n_pad] each);
 // the product lands in a's planes.
 void launch_ntt_rows_product(const Shape& s, const DeviceTables& t, uint32_t* planes, const uint32_t* bplanes,
                              hipStream_t stream);
-// Inverse row-spanning stages of plane 0; writes v_k into the CRT operand tiles
-// [n_pad/256][r_pad/16][256][64] (signed bytes).
+// Inverse row-spanning stages of plane 0 for column strips [strip_begin,
+// strip_end) (defaults: every column); writes v_k into the CRT operand tiles
+// [n_pad/256][r_pad/16][256][64] (signed bytes) of those columns.
 void launch_ntt_inverse_columns(const Shape& s, const DeviceTables& t, uint32_t* planes, int8_t* rtiles,
-                                hipStream_t stream);
-// Coefficient tiles [tile0, tile0 + tiles), clamped to the plan's (default: all).
-void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, hipStream_t stream,
-                uint32_t tile0 = 0, uint32_t tiles = ~0u);
-// Coefficients [i_begin, i_begin + count), i_begin a multiple of 64.  When
-// reconstruct_writes_once(s), c is only written (each word once), so it may be
-// pinned host memory; otherwise c is read back and must be device memory.
+                                hipStream_t stream, uint32_t strip_begin = 0, uint32_t strip_end = ~0u);
+void launch_crt(const Shape& s, const DeviceTables& t, const int8_t* rtiles, int64_t* words, const TileSet& ts,
+                hipStream_t stream);
+// Reconstructs the coefficients of the tiles of `ts`: output row k*256 + u of c
+// (L words each) is coefficient tile_at(ts, k)*256 + u; rows of coefficients at
+// or past N are not written.  When reconstruct_writes_once(s), each word of c
+// is written once, so c may well be pinned host memory; otherwise (q wider than
+// 62 limbs) a coefficient whose quotient estimate was off is written again by
+// a later pass.
 inline bool reconstruct_writes_once(const Shape& s) { return s.t_words <= 64; }
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
-                        uint32_t* c, uint32_t i_begin, uint32_t count, hipStream_t stream);
+                        uint32_t* c, const TileSet& ts, hipStream_t stream);
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream);
 
 // Plan-time table construction on the device (plans built with Tables::kDevice):

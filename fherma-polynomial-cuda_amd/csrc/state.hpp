@@ -193,18 +193,26 @@ struct pm_plan {
         }
     }
 
-    pm::Chunking chunking() const { return pm::make_chunking(host.shape.n_pad, host.shape.n, host.shape.limbs); }
-    // First half of a host-vector product.  Per coefficient chunk, the copy
-    // workers move a and b (one job) from the caller's pageable vectors into
-    // pinned staging as [a rows | b rows], one H2D carries both on a copy
-    // stream, and the compute stream waits on an event before packing and
-    // converting that chunk: the host copy of chunk k+1 overlaps the H2D of
-    // chunk k and the conversion of chunk k-1.  After the transforms and the CRT
-    // GEMM, each chunk is reconstructed and reaches pinned memory while the next
-    // is reconstructed: written there by the kernel itself when it stores each
-    // word once (short q), else by a D2H on the second copy stream.  Each DMA
-    // and each hand-off between the compute stream and a copy stream costs some
-    // 10-40 us on this link, which is why both are kept few.
+    pm::Chunking chunking() const {
+        const pm::Shape& s = host.shape;
+        return pm::make_chunking(s.n_pad, s.n, s.limbs, s.row_width, s.rows);
+    }
+    // First half of a host-vector product, per coefficient chunk (csrc/chunks.hpp):
+    // the copy workers gather the chunk's rows of a and b (one job) from the
+    // caller's pageable vectors into pinned staging as [a rows | b rows], one H2D
+    // carries both on a copy stream, and the compute stream waits on an event
+    // before packing and converting that chunk -- the host copy of chunk k+1
+    // overlaps the H2D of chunk k and the conversion of chunk k-1.  When the
+    // chunks are column strips, a strip's forward column stages follow the
+    // chunk that completes it, under the later chunks' transfers; the row stage
+    // needs every coefficient.  On the way out, per chunk: the inverse column
+    // stages of the strips it starts, its CRT GEMM and its reconstruction, which
+    // reaches pinned memory while the next chunk is computed -- written there by
+    // the kernel itself when it stores each word once (short q), else by a D2H
+    // on the second copy stream.  Each DMA and each hand-off between the compute
+    // stream and a copy stream costs some 10-40 us on this link, which is why
+    // both are kept few.  Staging, the device copies of the operands and of c
+    // hold the chunks one after another (chunk-major rows).
     // Returns with the device busy; multiply_finish collects c.
     // b == nullptr: the b residues come from `bplanes_in` (a resident operand).
     void multiply_begin(const uint32_t* a, const uint32_t* b, const uint32_t* bplanes_in) {
@@ -233,68 +241,69 @@ struct pm_plan {
         try {
             const pm::Chunking chunks = chunking();
             for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-                size_t r0, r1;
-                chunks.rows(ch, s.n, r0, r1);
                 // the chunk's rows of a, then (two operands) its rows of b, at word
-                // offset nops*r0*L in both staging and io_a
-                const size_t base = nops * r0 * L, rows = r1 - r0;
-                if (rows) {
-                    const pm::CopyPool::Segment segs[2] = {{lane->pin_in + base, a + r0 * L, rows * row_bytes},
-                                                           {lane->pin_in + base + rows * L, b ? b + r0 * L : nullptr,
-                                                            rows * row_bytes}};
-                    if (ch == 0 && nops == 2) {
-                        // Nothing overlaps the first chunk's host copy, so it goes in
-                        // two halves: a's rows, whose H2D then runs while b's rows
-                        // are copied, then b's rows.
-                        for (uint32_t op = 0; op < 2; ++op) {
-                            pool->copy(&segs[op], 1);
-                            hip_check(hipMemcpyAsync(io_a + base + op * rows * L, segs[op].dst, rows * row_bytes,
-                                                     hipMemcpyHostToDevice, copy_in),
-                                      "H2D inputs");
-                        }
-                    } else {
-                        pool->copy(segs, nops);
-                        hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
-                                                 hipMemcpyHostToDevice, copy_in),
+                // offset nops*local*L in both staging and io_a
+                const size_t rows = chunks.rows_of(ch), base = size_t(nops) * chunks.local[ch] * L;
+                const uint32_t runs = chunks.runs(ch);
+                if (runs > pm::kMaxRuns) throw std::logic_error("a chunk of more runs than kMaxRuns");
+                pm::CopyPool::Segment segs[2 * pm::kMaxRuns];
+                for (uint32_t op = 0; op < nops; ++op)
+                    for (uint32_t g = 0; g < runs; ++g) {
+                        size_t global0, local0, len;
+                        chunks.run(ch, g, s.n, global0, local0, len);
+                        segs[op * runs + g] = {lane->pin_in + base + (op * rows + local0 - chunks.local[ch]) * L,
+                                               (op ? b : a) + global0 * L, len * row_bytes};
+                    }
+                if (ch == 0 && nops == 2) {
+                    // Nothing overlaps the first chunk's host copy, so it goes in
+                    // two halves: a's rows, whose H2D then runs while b's rows are
+                    // copied, then b's rows.
+                    for (uint32_t op = 0; op < 2; ++op) {
+                        pool->copy(segs + op * runs, runs);
+                        hip_check(hipMemcpyAsync(io_a + base + op * rows * L, lane->pin_in + base + op * rows * L,
+                                                 rows * row_bytes, hipMemcpyHostToDevice, copy_in),
                                   "H2D inputs");
                     }
+                } else {
+                    pool->copy(segs, size_t(nops) * runs);
+                    hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
+                                             hipMemcpyHostToDevice, copy_in),
+                              "H2D inputs");
                 }
                 hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
-                uint32_t t0, tc;
-                chunks.tiles(ch, t0, tc);
-                if (tc) {
-                    // pack reads coefficient i at x + i*L: offset the bases so rows
-                    // r0.. land on the chunk's a and b blocks
-                    const uint32_t* xa = io_a + (nops == 2 ? r0 * L : 0);
-                    const uint32_t* xb = io_a + r1 * L;
-                    pm::launch_pack(s, xa, xb, work.packed, t0, tc, 0, nops, stream);
-                    pm::launch_conversion(s, tables, work.packed, work.planes, t0, tc, 0, nops, stream);
-                }
+                const uint32_t* xa = io_a + base;  // pack reads the chunk's k-th tile at rows k*256..
+                const uint32_t* xb = xa + rows * L;
+                pm::launch_pack(s, xa, xb, work.packed, chunks.set[ch], 0, nops, stream);
+                pm::launch_conversion(s, tables, work.packed, work.planes, chunks.set[ch], 0, nops, stream);
+                if (chunks.strips && chunks.fwd_end[ch] > chunks.fwd_begin[ch])
+                    pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, stream, chunks.fwd_begin[ch],
+                                                   chunks.fwd_end[ch]);
             }
-            run_transforms(stream, nops, bplanes);
-            // Output half, per chunk: the CRT GEMM of the chunk's coefficient
-            // tiles, its reconstruction, then its D2H on the copy stream, which
-            // overlaps the next chunk's GEMM.
+            if (!chunks.strips) pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, stream);
+            pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, stream);
+            if (!chunks.strips) pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, stream);
+            // Output half, per chunk: the inverse column stages of the strips it
+            // starts, its CRT GEMM, its reconstruction, then its D2H on the copy
+            // stream, which overlaps the next chunk's GEMM.
             const bool direct = pm::reconstruct_writes_once(s);
             for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-                size_t r0, r1;
-                chunks.rows(ch, s.n, r0, r1);
-                uint32_t t0, tc;
-                chunks.tiles(ch, t0, tc);
-                pm::launch_crt(s, tables, work.rtiles, work.words, stream, t0, tc);
-                pm::launch_reconstruct(s, tables, work.words, work.t_words, direct ? lane->pin_out : io_c, uint32_t(r0),
-                                       uint32_t(r1 - r0), stream);
+                const size_t rows = chunks.rows_of(ch), at = size_t(chunks.local[ch]) * L;
+                if (chunks.strips && chunks.inv_end[ch] > chunks.inv_begin[ch])
+                    pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, stream, chunks.inv_begin[ch],
+                                                   chunks.inv_end[ch]);
+                pm::launch_crt(s, tables, work.rtiles, work.words, chunks.set[ch], stream);
+                pm::launch_reconstruct(s, tables, work.words, work.t_words, (direct ? lane->pin_out : io_c) + at,
+                                       chunks.set[ch], stream);
                 if (direct) {
                     hip_check(hipEventRecord(lane->out_ready[ch], stream), "hipEventRecord");
                     continue;
                 }
                 hip_check(hipEventRecord(lane->reconstructed[ch], stream), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(copy_out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
-                if (r1 > r0)
-                    hip_check(hipMemcpyAsync(lane->pin_out + r0 * L, io_c + r0 * L, (r1 - r0) * row_bytes,
-                                             hipMemcpyDeviceToHost, copy_out),
-                              "D2H c");
+                hip_check(hipMemcpyAsync(lane->pin_out + at, io_c + at, rows * row_bytes, hipMemcpyDeviceToHost,
+                                         copy_out),
+                          "D2H c");
                 hip_check(hipEventRecord(lane->out_ready[ch], copy_out), "hipEventRecord");
             }
         } catch (...) {
@@ -305,8 +314,8 @@ struct pm_plan {
         idle.armed = false;  // the pool keeps spinning until multiply_finish
     }
 
-    // Second half: waits for each chunk's D2H and copies it into c (c == nullptr:
-    // waits and discards the result).
+    // Second half: waits for each chunk's results and scatters them into c
+    // (c == nullptr: waits and discards the result).
     void multiply_finish(uint32_t* c) {
         if (!pending) throw std::invalid_argument("no product has been begun on this plan");
         pending = false;
@@ -322,10 +331,16 @@ struct pm_plan {
         } idle{lane->pool.get()};
         const pm::Chunking chunks = chunking();
         for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-            size_t r0, r1;
-            chunks.rows(ch, s.n, r0, r1);
             hip_check(hipEventSynchronize(lane->out_ready[ch]), "D2H c");
-            if (c && r1 > r0) lane->pool->copy(c + r0 * L, lane->pin_out + r0 * L, (r1 - r0) * row_bytes);
+            if (!c) continue;
+            const uint32_t runs = std::min(chunks.runs(ch), pm::kMaxRuns);  // multiply_begin checked it
+            pm::CopyPool::Segment segs[pm::kMaxRuns];
+            for (uint32_t g = 0; g < runs; ++g) {
+                size_t global0, local0, len;
+                chunks.run(ch, g, s.n, global0, local0, len);
+                segs[g] = {c + global0 * L, lane->pin_out + local0 * L, len * row_bytes};
+            }
+            lane->pool->copy(segs, runs);
         }
         hip_check(hipStreamSynchronize(lane->stream), "multiply");
     }
@@ -459,13 +474,13 @@ struct pm_plan {
         auto mark = [&](int i) {
             if (profiling) hip_check(hipEventRecord(lane->marks[i], st), "hipEventRecord");
         };
-        const uint32_t ntiles = s.n_pad / 256;
+        const pm::TileSet all = pm::all_tiles(s);
         const uint32_t nops = b ? 2 : 1;
         const uint32_t* bplanes = b ? b_planes(w) : bplanes_in;
         mark(0);
-        pm::launch_pack(s, a, b, w.packed, 0, ntiles, 0, nops, st);
+        pm::launch_pack(s, a, b, w.packed, all, 0, nops, st);
         mark(1);
-        pm::launch_conversion(s, tables, w.packed, w.planes, 0, ntiles, 0, nops, st);
+        pm::launch_conversion(s, tables, w.packed, w.planes, all, 0, nops, st);
         mark(2);
         if (last == PM_STAGE_RESIDUES) return;
         pm::launch_ntt_forward_columns(s, tables, w.planes, 0, nops, st);
@@ -475,20 +490,11 @@ struct pm_plan {
         pm::launch_ntt_inverse_columns(s, tables, w.planes, w.rtiles, st);
         mark(5);
         if (last == PM_STAGE_PRODUCT) return;
-        pm::launch_crt(s, tables, w.rtiles, w.words, st);
+        pm::launch_crt(s, tables, w.rtiles, w.words, all, st);
         mark(6);
         if (last == PM_STAGE_WORDS) return;
-        pm::launch_reconstruct(s, tables, w.words, w.t_words, c, 0, s.n, st);
+        pm::launch_reconstruct(s, tables, w.words, w.t_words, c, all, st);
         mark(7);
-    }
-
-    // The stages between the per-coefficient ends: the transforms, which couple
-    // every coefficient of a column.
-    void run_transforms(hipStream_t st, uint32_t nops, const uint32_t* bplanes) {
-        const pm::Shape& s = host.shape;
-        pm::launch_ntt_forward_columns(s, tables, work.planes, 0, nops, st);
-        pm::launch_ntt_rows_product(s, tables, work.planes, bplanes, st);
-        pm::launch_ntt_inverse_columns(s, tables, work.planes, work.rtiles, st);
     }
 
     void collect_times() {

@@ -252,3 +252,31 @@ def test_wide_q_reconstruction_edges(polymul, workloads, oracle, n, limbs, kind)
             _check_product(plan, e, b, q, oracle)
         m = _edge_operands(workloads, q, n, "max")
         _check_product(plan, m, m, q, oracle)
+
+
+# Host products large enough for one chunk per column strip move their
+# coefficients in strips (256 columns of the [rows][2048] matrix the transforms
+# use): 2 rows (one chunk per strip), 4 and 8 rows (the first and last strips
+# in pieces of rows).  The chunk-major staging, the per-strip column stages and
+# the scatter into c must give the device entry's result, which holds the
+# operands in their natural order; sampled coefficients against the oracle.
+@pytest.mark.parametrize("n,limbs", [(4096, 2700), (8192, 1350), (16384, 700)])
+def test_column_strip_chunks(polymul, workloads, oracle, device_memory, n, limbs):
+    _announce("column strips", n, limbs)
+    rng = np.random.default_rng(n + limbs)
+    q = workloads.random_modulus(limbs, rng)
+    a = workloads.uniform_below(q, n, rng)
+    b = workloads.uniform_below(q, n, rng)
+    with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+        c = plan.multiply(a, b)
+        da, db = device_memory.upload(a), device_memory.upload(b)
+        dc = device_memory.alloc(a.nbytes)
+        try:
+            plan.multiply_device(da, db, dc)
+            assert np.array_equal(device_memory.download(dc, a.shape, np.uint32), c)
+        finally:
+            for ptr in (da, db, dc):
+                device_memory.free(ptr)
+        for i in (0, 255, 256, 2047, 2048, n // 2 + 300, n - 1):
+            assert np.array_equal(c[i], oracle.coefficient_gmp(a, b, q, i)), "coefficient %d" % i
+        assert np.array_equal(plan.multiply(b, a), c)

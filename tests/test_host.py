@@ -40,11 +40,14 @@ def test_device_arena_with_a_fake_device(tmp_path):
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ to build the chunk check")
 def test_pipeline_chunks_stay_inside_the_tiles(tmp_path):
-    """Every chunk of the host pipeline covers a non-empty tile range inside
-    [0, ntiles), for every power-of-two N up to 2^24 and L up to 8192 limbs
-    (tests/chunk_check.cpp).  A chunk starting past the last tile used to make
-    the packing kernel write past the packed-operand buffer (e.g. N=2048,
-    L=4096: 6 chunks of 2 tiles over 8 tiles)."""
+    """Every chunk of the host pipeline is a non-empty set of tiles inside
+    [0, ntiles), the chunks cover every tile and every coefficient once, and
+    with column strips each strip's column stages are issued once, after its
+    last piece is in and before its first piece goes out -- for every
+    power-of-two N up to 2^24 and L up to 8192 limbs (tests/chunk_check.cpp).
+    A chunk starting past the last tile used to make the packing kernel write
+    past the packed-operand buffer (e.g. N=2048, L=4096: 6 chunks of 2 tiles
+    over 8 tiles)."""
     root = Path(__file__).resolve().parent.parent
     csrc = root / "fherma-polynomial-cuda_amd" / "csrc"
     exe = tmp_path / "chunk_check"
