@@ -10,7 +10,7 @@
 
 namespace pm {
 
-constexpr int kMaxChunks = 12;
+constexpr int kMaxChunks = 16;
 // Runs of consecutive tiles in one chunk: at most the 64 rows of a strip.
 constexpr uint32_t kMaxRuns = 64;
 
@@ -69,13 +69,20 @@ struct Chunking {
 // registers (2 to 64 rows of a multiple of 256 columns, so at most 8 strips)
 // and at least one chunk per strip is wanted: the forward column stages of a
 // strip then run while later strips are still on the link, and the inverse
-// column stages of a strip run while earlier strips' results go out.  The
-// first and the last strip go in three pieces of rows (a quarter, a quarter, a
-// half; and a half, a quarter, a quarter) when there are 4 rows or more: the
-// first piece's host copy comes before any transfer starts, and after the last
-// transfer the last piece's conversion, the transforms of its strip and the
-// rows, and on the way out the first piece's CRT GEMM run with the link idle --
-// the smaller those pieces, the less of that is exposed.  (Chunks of several
+// column stages of a strip run while earlier strips' results go out.  With 4
+// rows or more the end strips go in pieces of rows: the first and the last in
+// three (a quarter, a quarter, a half; a half, a quarter, a quarter), the
+// second and the second to last in halves, the rest whole (at 8 strips: 14
+// chunks).  The first piece's host copy comes before any transfer starts, and
+// after the last transfer the last piece's conversion, its strip's forward
+// column stages and the row stage, and on the way out the first piece's
+// inverse column stages and CRT GEMM, run with the link idle -- the smaller
+// those pieces, the less of that is exposed.  The halves keep the compute
+// stream from trailing the link: a chunk's conversion starts when its transfer
+// ends, so a whole strip just before the last one (its conversion takes 0.68
+// of its transfer time) would delay the last pieces' conversions; and on the
+// way out a strip's inverse stages, CRT GEMM and reconstruction (0.83 of its
+// D2H time) must fit under the previous chunk's D2H.  (Chunks of several
 // whole strips, for fewer chunks, measured slower than tile ranges at
 // N = 65536 with 55 limbs: 1.35-1.38 against 1.28-1.29 ms per product.)
 //
@@ -107,7 +114,14 @@ inline Chunking make_chunking(uint32_t n_pad, uint32_t n, uint32_t limbs, uint32
             piece(0, 0, rows / 4);
             piece(0, rows / 4, rows / 2);
             piece(0, rows / 2, rows);
-            for (uint32_t s = 1; s + 1 < S; ++s) piece(s, 0, rows);
+            for (uint32_t s = 1; s + 1 < S; ++s) {
+                if (S >= 4 && (s == 1 || s == S - 2)) {
+                    piece(s, 0, rows / 2);
+                    piece(s, rows / 2, rows);
+                } else {
+                    piece(s, 0, rows);
+                }
+            }
             piece(S - 1, 0, rows / 2);
             piece(S - 1, rows / 2, rows / 2 + rows / 4);
             piece(S - 1, rows / 2 + rows / 4, rows);
