@@ -131,19 +131,19 @@ def batched_leg(polymul, workloads, count, lanes, passes=3):
             "host_link_gb": round(12 * words / 1e9, 2)}, products
 
 
-def device_batch_leg(polymul, torch, products, lanes, passes=3):
-    """The batched workload with every operand already in HBM, apart from the
+def device_batch_leg(polymul, torch, products, lanes, passes=3, golden_key="batch"):
+    """A batched workload with every operand already in HBM, apart from the
     host link and host memory: all 64 plans are built first (device tables)
     and the operands uploaded.  Plain runs pm_multiply_device one product after
     another; batched runs `lanes` host threads, each calling pm_multiply_device
     on its next product (largest first, by N*L^2), so the products' kernel
     sequences overlap on the device through the plans' own streams.  Every
     output is hashed and compared with the oracle's digest of the workload
-    (tests/golden/workload_digests.json)."""
+    (tests/golden/workload_digests.json, entry `golden_key`)."""
     import hashlib
     import threading
 
-    golden = json.loads((ROOT / "tests" / "golden" / "workload_digests.json").read_text())["batch"]["products"]
+    golden = json.loads((ROOT / "tests" / "golden" / "workload_digests.json").read_text())[golden_key]["products"]
     dev = torch.device("cuda:0")
     plans, bufs = [], []
     try:
@@ -727,8 +727,11 @@ def main():
         plain_products(polymul, warm)
         batched, products = batched_leg(polymul, workloads, args.batch_count, args.lanes)
         result["batched"] = batched
-        if args.batch_count == 64:  # the golden digests cover the 64-product workload
+        if args.batch_count == 64:  # the golden digests cover the 64-product workloads
             result["batched_device"] = device_batch_leg(polymul, torch, products, args.lanes)
+            # the mix of small products (N <= 8192, L <= 128), none of which fills the GPU alone
+            result["batched_device_small"] = device_batch_leg(polymul, torch, workloads.batch_small_products(),
+                                                              args.lanes, golden_key="batch_small")
         del products
         result["resident_operand"] = resident_leg(polymul, workloads, 64)
         capacity = batched["batched_products_per_s"]

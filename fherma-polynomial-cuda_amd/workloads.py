@@ -129,12 +129,22 @@ POWERS = [4096, 8192, 16384, 32768, 65536]
 # [32, 1747].  The resident-operand workload: 64 random a times one fixed b at
 # N = 65536, L = 55.
 BATCH = {"count": 64, "seed": 0x5EED0004, "limbs": (16, 1747)}
+# A mix of small products only (N in {2^10..2^13}, L log-uniform in [8, 128]),
+# none of which fills the GPU alone: what running products side by side gains
+# when the device is not already saturated by the largest of them.
+SMALL_POWERS = [1024, 2048, 4096, 8192]
+BATCH_SMALL = {"count": 64, "seed": 0x5EED0007, "limbs": (8, 128)}
 SERVED = {"count": 48, "seed": 0x5EED0006, "limbs": (32, 1747)}
 RESIDENT = {"count": 64, "seed": 0x5EED0005, "n": 65536, "limbs": 55}
 
 
 def batch_products(count=None):
     return make_products(BATCH["count"] if count is None else count, BATCH["seed"], POWERS, BATCH["limbs"])
+
+
+def batch_small_products(count=None):
+    return make_products(BATCH_SMALL["count"] if count is None else count, BATCH_SMALL["seed"], SMALL_POWERS,
+                         BATCH_SMALL["limbs"])
 
 
 def served_products(count=None):

@@ -10,6 +10,7 @@ SHA-256 digests of the whole result:
   large     N=65536, L=1747 (55,904-bit random odd q): digest of c, plus one
             digest per block of 1024 coefficients to localize a mismatch
   batch     the 64 products of the batched workload (one digest each)
+  batch_small  the 64 products of the small-product mix
   served    the 48 requests of the served workload
   resident  the 64 products a_i * b of the resident-operand workload
   stress    the reference's fresh-seed stress cases (tools/stress_local.py:16-21):
@@ -108,7 +109,7 @@ def products(items, jobs):
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--jobs", type=int, default=4, help="oracle processes (each up to ~12 GB for the largest)")
-    parser.add_argument("--only", default="large,batch,served,resident,stress")
+    parser.add_argument("--only", default="large,batch,batch_small,served,resident,stress")
     args = parser.parse_args()
     w = _workloads()
     data = json.loads(OUT.read_text()) if OUT.is_file() else {}
@@ -118,6 +119,9 @@ def main():
             data["large"] = large()
         elif part == "batch":
             data["batch"] = {"seed": w.BATCH["seed"], "products": products(w.batch_products(), args.jobs)}
+        elif part == "batch_small":
+            data["batch_small"] = {"seed": w.BATCH_SMALL["seed"],
+                                   "products": products(w.batch_small_products(), args.jobs)}
         elif part == "served":
             data["served"] = {"seed": w.SERVED["seed"], "products": products(w.served_products(), args.jobs)}
         elif part == "resident":

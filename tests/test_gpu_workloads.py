@@ -8,6 +8,9 @@ tests/golden/workload_digests.json.
                    host-vector entry and the device entry
   batched          the 64 products of the batched workload through
                    pm_multiply_batch (60 GB budget, 4 lanes)
+  small batched    64 small products (N <= 8192, L <= 128) through
+                   pm_multiply_batch and through pm_multiply_device from
+                   four threads at once
   resident         64 random a times one fixed b, b's residues on the device
   served           the 48 requests of the served workload with Poisson
                    arrivals, 200 GB budget: each result in the mix must be the
@@ -76,6 +79,57 @@ def test_batched_workload(polymul, workloads):
     outs = polymul.multiply_batch(products, 60 * GiB, lanes=4)
     bad = [i for i, (c, e) in enumerate(zip(outs, g)) if workloads.digest(c) != e["c"]]
     assert not bad, "batched products differ from the oracle: %s" % [(i, g[i]["n"], g[i]["limbs"]) for i in bad]
+
+
+def test_small_batched_workload(polymul, workloads, device_memory):
+    """The mix of 64 small products (N <= 8192, L <= 128) through
+    pm_multiply_batch, and through pm_multiply_device from four threads at
+    once (each plan on its own stream), against the oracle's digests."""
+    import threading
+
+    g = GOLDEN["batch_small"]["products"]
+    products = workloads.batch_small_products()
+    assert len(products) == len(g)
+    _announce("small batched workload", len(products))
+    for (n, limbs, q, a, b), e in zip(products, g):
+        _check_inputs(workloads, e, q, a, b, "small batch product N=%d L=%d" % (n, limbs))
+    outs = polymul.multiply_batch(products, 4 * GiB, lanes=4)
+    bad = [i for i, (c, e) in enumerate(zip(outs, g)) if workloads.digest(c) != e["c"]]
+    assert not bad, "batched products differ from the oracle: %s" % bad
+    plans = [polymul.Plan(n, 32 * limbs, limbs, q) for n, limbs, q, _a, _b in products]
+    bufs = [(device_memory.upload(a), device_memory.upload(b), device_memory.alloc(a.nbytes))
+            for _n, _l, _q, a, b in products]
+    try:
+        pending = iter(range(len(products)))
+        lock = threading.Lock()
+        errors = []
+
+        def worker():
+            while True:
+                with lock:
+                    i = next(pending, None)
+                if i is None:
+                    return
+                try:
+                    plans[i].multiply_device(*bufs[i])
+                except Exception as e:  # reported below, in the test's thread
+                    errors.append((i, e))
+
+        threads = [threading.Thread(target=worker) for _ in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        assert not errors, errors
+        bad = [i for i, ((n, limbs, *_r), (_a, _b, pc), e) in enumerate(zip(products, bufs, g))
+               if workloads.digest(device_memory.download(pc, (n, limbs), np.uint32)) != e["c"]]
+        assert not bad, "device products from four threads differ from the oracle: %s" % bad
+    finally:
+        for pa, pb, pc in bufs:
+            for ptr in (pa, pb, pc):
+                device_memory.free(ptr)
+        for plan in plans:
+            plan.close()
 
 
 def test_resident_operand_workload(polymul, workloads):
