@@ -137,7 +137,9 @@ def device_batch_leg(polymul, torch, products, lanes, passes=3, golden_key="batc
     and the operands uploaded.  Plain runs pm_multiply_device one product after
     another; batched runs `lanes` host threads, each calling pm_multiply_device
     on its next product (largest first, by N*L^2), so the products' kernel
-    sequences overlap on the device through the plans' own streams.  Every
+    sequences overlap on the device through the plans' own streams; grouped
+    is one pm_multiply_device_batch call, each stage one launch for all
+    products.  Every
     output is hashed and compared with the oracle's digest of the workload
     (tests/golden/workload_digests.json, entry `golden_key`)."""
     import hashlib
@@ -199,19 +201,31 @@ def device_batch_leg(polymul, torch, products, lanes, passes=3, golden_key="batc
                 out.append(hashlib.sha256(np.ascontiguousarray(c, dtype="<u4").tobytes()).hexdigest())
             return out
 
+        pointers = [[t.data_ptr() for t in column] for column in zip(*bufs)]
+
+        def grouped():
+            polymul.multiply_device_batch(plans, *pointers)
+
         plain_s = timed(plain)
         plain_ok = digests() == [g["c"] for g in golden]
         for _, _, tc in bufs:
             tc.zero_()
         batched_s = timed(batched)
         batched_ok = digests() == [g["c"] for g in golden]
+        for _, _, tc in bufs:
+            tc.zero_()
+        grouped_s = timed(grouped)
+        grouped_ok = digests() == [g["c"] for g in golden]
         count = len(products)
         return {"products": count, "lanes": lanes, "timed_passes": passes,
                 "plain_products_per_s": round(count / plain_s, 2),
                 "batched_products_per_s": round(count / batched_s, 2),
                 "batched_vs_plain": round(plain_s / batched_s, 2),
+                "grouped_products_per_s": round(count / grouped_s, 2),
+                "grouped_vs_plain": round(plain_s / grouped_s, 2),
                 "plain_ms": round(plain_s * 1000.0, 1), "batched_ms": round(batched_s * 1000.0, 1),
-                "exact_vs_oracle_digests": bool(plain_ok and batched_ok)}
+                "grouped_ms": round(grouped_s * 1000.0, 2),
+                "exact_vs_oracle_digests": bool(plain_ok and batched_ok and grouped_ok)}
     finally:
         for plan in plans:
             plan.close()

@@ -6,6 +6,8 @@
 // every q instead of the reference's FP64 fallbacks (solve.cu:718-895).
 #include "polymul.h"
 
+#include <algorithm>
+
 #include "state.hpp"
 
 thread_local std::string g_last_error;
@@ -91,6 +93,62 @@ pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b
         plan->run(a, b, c, PM_STAGE_OUTPUT, st);
         hip_check(hipStreamSynchronize(st), "multiply");
         plan->collect_times();
+    });
+}
+
+pm_status pm_multiply_device_batch(pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                                   uint32_t* const* c, size_t count, void* stream) {
+    return guarded([&] {
+        if (count == 0) return;
+        if (!plans || !a || !b || !c) throw std::invalid_argument("null argument");
+        std::vector<const pm_plan*> seen;
+        seen.reserve(count);
+        int device = -1;
+        for (size_t i = 0; i < count; ++i) {
+            if (!plans[i]) throw std::invalid_argument("null plan");
+            require_device(plans[i]);
+            require_idle(plans[i]);
+            if (!a[i] || !b[i] || !c[i]) throw std::invalid_argument("null operand");
+            if (device < 0) device = plans[i]->device_id;
+            if (plans[i]->device_id != device) throw std::invalid_argument("plans of one batch on different devices");
+            seen.push_back(plans[i]);
+        }
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            throw std::invalid_argument("a plan appears twice in one batch (its working buffers hold one product)");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plans[0]->lane->stream;
+        std::vector<pm::GroupItem> items;
+        for (size_t i = 0; i < count; ++i) {
+            const pm::Shape& s = plans[i]->host.shape;
+            if (pm::groupable(s))
+                items.push_back(pm::group_item(s, plans[i]->tables, plans[i]->work, a[i], b[i], c[i]));
+            else
+                plans[i]->run(a[i], b[i], c[i], PM_STAGE_OUTPUT, st);  // on its own, ahead of the grouped launches
+        }
+        if (!items.empty()) {
+            // the items and the launches' work lists, in one buffer for the call
+            const size_t item_bytes = (items.size() * sizeof(pm::GroupItem) + 255) / 256 * 256;
+            const size_t table_bytes = pm::group_table_words(items.size()) * sizeof(uint32_t);
+            void* mem = nullptr;
+            hip_check(hipMalloc(&mem, item_bytes + table_bytes), "hipMalloc");
+            struct Release {
+                void* p;
+                ~Release() { (void)hipFree(p); }
+            } release{mem};
+            try {
+                hip_check(hipMemcpy(mem, items.data(), items.size() * sizeof(pm::GroupItem), hipMemcpyHostToDevice),
+                          "upload batch");
+                pm::launch_group(items, static_cast<const pm::GroupItem*>(mem),
+                                 reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + item_bytes), st);
+            } catch (...) {
+                (void)hipStreamSynchronize(st);  // nothing queued may outlive the buffer
+                throw;
+            }
+            hip_check(hipStreamSynchronize(st), "multiply batch");
+        } else {
+            hip_check(hipStreamSynchronize(st), "multiply batch");
+        }
     });
 }
 

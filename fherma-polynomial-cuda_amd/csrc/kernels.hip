@@ -28,6 +28,7 @@
 // 256, so every GEMM tile load is in bounds and 16-byte aligned.
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -120,8 +121,9 @@ __device__ __forceinline__ uint32_t swz(uint32_t row, uint32_t chunk) {
 // walk that range in column-major groups of kGroupM tile rows: the blocks an
 // XCD runs together share A tiles (across a column) and B tiles (across a
 // group row) in that XCD's L2.
-__device__ __forceinline__ void tile_coords(uint32_t num_m, uint32_t num_n, uint32_t& tm, uint32_t& tn) {
-    const uint32_t bid = blockIdx.x, total = num_m * num_n;
+__device__ __forceinline__ void tile_coords(uint32_t bid, uint32_t num_m, uint32_t num_n, uint32_t& tm,
+                                            uint32_t& tn) {
+    const uint32_t total = num_m * num_n;
     const uint32_t per = total / 8;
     const uint32_t logical = bid < per * 8 ? (bid % 8) * per + bid / 8 : bid;
     const uint32_t in_group = kGroupM * num_n;
@@ -242,14 +244,13 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
 // Packs the tiles of `ts` of operands op0 .. op0+nops-1.  The k-th tile's rows
 // are read from rows k*256 .. k*256+255 of a (or b): the caller passes the
 // operands' rows in the set's order (all of [N][L] for the set of every tile).
-__global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict__ a,
-                                                     const uint32_t* __restrict__ b,
-                                                     uint32_t* __restrict__ packed, uint32_t n, uint32_t limbs,
-                                                     uint32_t ksteps, uint32_t ntiles, TileSet ts, uint32_t op0,
-                                                     uint32_t nops) {
+__device__ __forceinline__ void pack_body(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                                          uint32_t* __restrict__ packed, uint32_t n, uint32_t limbs,
+                                          uint32_t ksteps, uint32_t ntiles, TileSet ts, uint32_t op0,
+                                          uint32_t nops, uint32_t bx) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t per_op = ts.count * kTile;
-    const uint32_t gr = blockIdx.x * 4 + (threadIdx.x >> 6);  // row over all operands of the launch
+    const uint32_t gr = bx * 4 + (threadIdx.x >> 6);  // row over all operands of the launch
     if (gr >= nops * per_op) return;
     const uint32_t op = op0 + gr / per_op, rr = gr % per_op;
     const uint32_t tn = tile_at(ts, rr / kTile), row = rr % kTile, coef = tn * kTile + row;
@@ -278,22 +279,29 @@ __global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict_
     }
 }
 
+__global__ __launch_bounds__(256) void pack_operands(const uint32_t* __restrict__ a,
+                                                     const uint32_t* __restrict__ b,
+                                                     uint32_t* __restrict__ packed, uint32_t n, uint32_t limbs,
+                                                     uint32_t ksteps, uint32_t ntiles, TileSet ts, uint32_t op0,
+                                                     uint32_t nops) {
+    pack_body(a, b, packed, n, limbs, ksteps, ntiles, ts, op0, nops, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // 1. conversion GEMM: G[4k+d][i] = sum_j digit_d(lambda*256^j mod p_k) * (byte_j(x_i)-128)
 //    then x_i*lambda mod p_k = REDC(sum_d 256^d G[4k+d][i] + off_k).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict__ wtiles,
-                                                       const int8_t* __restrict__ packed,
-                                                       const PrimeConst* __restrict__ consts,
-                                                       uint32_t* __restrict__ planes, uint32_t n, uint32_t n_pad,
-                                                       uint32_t r, uint32_t r_pad, uint32_t conv_rows,
-                                                       uint32_t ksteps, TileSet ts, uint32_t op0, uint32_t nops) {
+__device__ __forceinline__ void conversion_body(const int8_t* __restrict__ wtiles, const int8_t* __restrict__ packed,
+                                                const PrimeConst* __restrict__ consts, uint32_t* __restrict__ planes,
+                                                uint32_t n, uint32_t n_pad, uint32_t r, uint32_t r_pad,
+                                                uint32_t conv_rows, uint32_t ksteps, TileSet ts, uint32_t op0,
+                                                uint32_t nops, uint32_t bid) {
     PM_GEMM_STAGES(stages);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t ntiles = n_pad / kTile, num_m = conv_rows / kTile;
     uint32_t tm, tn;
-    tile_coords(num_m, nops * ts.count, tm, tn);  // the set's coefficient tiles of nops operands
+    tile_coords(bid, num_m, nops * ts.count, tm, tn);  // the set's coefficient tiles of nops operands
     const uint32_t op = op0 + tn / ts.count, tcol = tile_at(ts, tn % ts.count);
     const int8_t* a_panel = wtiles + (size_t(op) * num_m + tm) * ksteps * kTileBytes;
     const int8_t* b_panel = packed + (size_t(op) * ntiles + tcol) * ksteps * kTileBytes;
@@ -322,6 +330,15 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
     }
 }
 
+__global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict__ wtiles,
+                                                       const int8_t* __restrict__ packed,
+                                                       const PrimeConst* __restrict__ consts,
+                                                       uint32_t* __restrict__ planes, uint32_t n, uint32_t n_pad,
+                                                       uint32_t r, uint32_t r_pad, uint32_t conv_rows,
+                                                       uint32_t ksteps, TileSet ts, uint32_t op0, uint32_t nops) {
+    conversion_body(wtiles, packed, consts, planes, n, n_pad, r, r_pad, conv_rows, ksteps, ts, op0, nops, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // 5. CRT GEMM: G[c][i] = sum_{k,d} w[c][4k+d] * (byte_d(v_{k,i}) - 128),
 //    D_j(i) = sum_t 256^t G[4j+t][i]   (int64, exact)
@@ -331,17 +348,16 @@ __global__ __launch_bounds__(512) void conversion_gemm(const int8_t* __restrict_
 //    runs per coefficient, written one after another.
 // ---------------------------------------------------------------------------
 template <bool kByCoefficient>
-__global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtiles,
-                                                const int8_t* __restrict__ rtiles,
-                                                int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
-                                                uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words,
-                                                uint32_t ws, TileSet ts) {
+__device__ __forceinline__ void crt_body(const int8_t* __restrict__ wtiles, const int8_t* __restrict__ rtiles,
+                                         int64_t* __restrict__ words, uint32_t n, uint32_t n_pad, uint32_t crt_rows,
+                                         uint32_t ksteps, uint32_t crt_words, uint32_t ws, TileSet ts,
+                                         uint32_t bid) {
     PM_GEMM_STAGES(stages);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t wm = wave / kWavesN, wn = wave % kWavesN;
     const uint32_t num_m = crt_rows / kTile;
     uint32_t tm, tn;
-    tile_coords(num_m, ts.count, tm, tn);  // the set's coefficient tiles
+    tile_coords(bid, num_m, ts.count, tm, tn);  // the set's coefficient tiles
     tn = tile_at(ts, tn);
     const int8_t* a_panel = wtiles + size_t(tm) * ksteps * kTileBytes;
     const int8_t* b_panel = rtiles + size_t(tn) * ksteps * kTileBytes;
@@ -380,6 +396,15 @@ __global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtile
     }
 }
 
+template <bool kByCoefficient>
+__global__ __launch_bounds__(512) void crt_gemm(const int8_t* __restrict__ wtiles,
+                                                const int8_t* __restrict__ rtiles,
+                                                int64_t* __restrict__ words, uint32_t n, uint32_t n_pad,
+                                                uint32_t crt_rows, uint32_t ksteps, uint32_t crt_words,
+                                                uint32_t ws, TileSet ts) {
+    crt_body<kByCoefficient>(wtiles, rtiles, words, n, n_pad, crt_rows, ksteps, crt_words, ws, ts, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // 2-4. NTT.  Flat index i = row*C + col.  Forward: Cooley-Tukey with the
 // negacyclic twist merged into psi^bitrev twiddles, m = 1, 2, .., n2/2 groups
@@ -416,17 +441,15 @@ __global__ __launch_bounds__(256) void ntt_columns_forward(uint32_t* __restrict_
         data[size_t(t / cw) * C + col0 + t % cw] = tile[t];
 }
 
-__global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ planes,
-                                                        const uint32_t* __restrict__ bplanes,
-                                                        const PrimeConst* __restrict__ consts,
-                                                        const uint2* __restrict__ twf,
-                                                        const uint2* __restrict__ twi, uint32_t ps,
-                                                        uint32_t n1, uint32_t n2, uint32_t C,
-                                                        uint32_t rows, uint32_t r) {
-    extern __shared__ uint32_t lds32[];  // A[C], B[C]
-    uint32_t* A = lds32;
+// Row stages for any row width C <= 2048 and block size n1, through LDS;
+// lds32 holds 2*C words.
+__device__ __forceinline__ void rows_product_body(uint32_t* __restrict__ planes, const uint32_t* __restrict__ bplanes,
+                                                  const PrimeConst* __restrict__ consts,
+                                                  const uint2* __restrict__ twf, const uint2* __restrict__ twi,
+                                                  uint32_t ps, uint32_t n1, uint32_t n2, uint32_t C, uint32_t rows,
+                                                  uint32_t* lds32, uint32_t row, uint32_t k) {
+    uint32_t* A = lds32;  // A[C], B[C]
     uint32_t* B = lds32 + C;
-    const uint32_t row = blockIdx.x, k = blockIdx.y;
     const uint32_t p = consts[k].p, pinv = consts[k].pinv;
     uint32_t* pa = planes + size_t(k) * ps + size_t(row) * C;
     const uint32_t* pb = bplanes + size_t(k) * ps + size_t(row) * C;
@@ -496,6 +519,17 @@ __global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ p
     for (uint32_t t = threadIdx.x; t < C; t += blockDim.x) pa[t] = A[t];
 }
 
+__global__ __launch_bounds__(256) void ntt_rows_product(uint32_t* __restrict__ planes,
+                                                        const uint32_t* __restrict__ bplanes,
+                                                        const PrimeConst* __restrict__ consts,
+                                                        const uint2* __restrict__ twf,
+                                                        const uint2* __restrict__ twi, uint32_t ps,
+                                                        uint32_t n1, uint32_t n2, uint32_t C,
+                                                        uint32_t rows, uint32_t r) {
+    extern __shared__ uint32_t lds32[];
+    rows_product_body(planes, bplanes, consts, twf, twi, ps, n1, n2, C, rows, lds32, blockIdx.x, blockIdx.y);
+}
+
 __global__ __launch_bounds__(256) void ntt_columns_inverse(uint32_t* __restrict__ plane,
                                                            const PrimeConst* __restrict__ consts,
                                                            const uint2* __restrict__ tw, uint32_t ps,
@@ -544,12 +578,11 @@ __device__ __forceinline__ void gs_butterfly(uint32_t& a, uint32_t& b, uint2 w, 
 // the column's ROWS values; the stage twiddles are uniform over the block.
 // Columns col0 + [0, 256 * gridDim.x).
 template <int ROWS>
-__global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restrict__ planes,
-                                                               const PrimeConst* __restrict__ consts,
-                                                               const uint2* __restrict__ tw, uint32_t ps,
-                                                               uint32_t n2, uint32_t C, uint32_t r, uint32_t op0,
-                                                               uint32_t col0) {
-    const uint32_t col = col0 + blockIdx.x * 256 + threadIdx.x, k = blockIdx.y, op = op0 + blockIdx.z;
+__device__ __forceinline__ void forward_reg_body(uint32_t* __restrict__ planes, const PrimeConst* __restrict__ consts,
+                                                 const uint2* __restrict__ tw, uint32_t ps, uint32_t n2, uint32_t C,
+                                                 uint32_t r, uint32_t op0, uint32_t col0, uint32_t bx, uint32_t by,
+                                                 uint32_t bz) {
+    const uint32_t col = col0 + bx * 256 + threadIdx.x, k = by, op = op0 + bz;
     const uint32_t p = consts[k].p;
     uint32_t* data = planes + (size_t(op) * r + k) * ps + col;
     const uint2* w = tw + size_t(k) * n2;
@@ -568,6 +601,15 @@ __global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restr
     for (int j = 0; j < ROWS; ++j) data[size_t(j) * C] = x[j];
 }
 
+template <int ROWS>
+__global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restrict__ planes,
+                                                               const PrimeConst* __restrict__ consts,
+                                                               const uint2* __restrict__ tw, uint32_t ps,
+                                                               uint32_t n2, uint32_t C, uint32_t r, uint32_t op0,
+                                                               uint32_t col0) {
+    forward_reg_body<ROWS>(planes, consts, tw, ps, n2, C, r, op0, col0, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // Inverse row-spanning stages for 16 primes x 16 columns per block, then the
 // residues v_{k,i} are written, XOR 0x80808080, into the CRT GEMM's operand
 // tiles [n_pad/256][r_pad/16][256 coefficients][16 primes x 4 bytes] through
@@ -575,12 +617,11 @@ __global__ __launch_bounds__(256) void ntt_columns_forward_reg(uint32_t* __restr
 // ROWS = 1 is a pure transpose (no stages).  Column groups g0 + [0, gridDim.x)
 // of 16 columns each.
 template <int ROWS>
-__global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t* __restrict__ plane,
-                                                                 const PrimeConst* __restrict__ consts,
-                                                                 const uint2* __restrict__ tw, uint32_t ps,
-                                                                 uint32_t n, uint32_t n2, uint32_t C, uint32_t r,
-                                                                 uint32_t ksteps, int8_t* __restrict__ tiles,
-                                                                 uint32_t g0) {
+__device__ __forceinline__ void inverse_tiles_body(const uint32_t* __restrict__ plane,
+                                                   const PrimeConst* __restrict__ consts,
+                                                   const uint2* __restrict__ tw, uint32_t ps, uint32_t n, uint32_t n2,
+                                                   uint32_t C, uint32_t r, uint32_t ksteps, int8_t* __restrict__ tiles,
+                                                   uint32_t g0, uint32_t bx, uint32_t by, uint32_t nx) {
     // lds[j][column][slot]: the four 16-byte groups of a column's 16 primes are
     // XOR-swizzled by column/2, so the transpose reads each 16-byte store as one
     // aligned 128-bit LDS read (8 lanes of a read pass: two columns, distinct
@@ -594,9 +635,8 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t*
     // linear id, so column groups 2m and 2m+1 are given to blocks 8 ids apart
     // (the same XCD when the grid's x extent is a multiple of 16): the second
     // read of each line hits that XCD's L2 instead of memory.
-    const uint32_t bx = blockIdx.x;
-    const uint32_t cx = (gridDim.x & 15u) == 0 ? (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u) : bx;
-    const uint32_t c0 = (g0 + cx) * 16, kg = blockIdx.y, k = kg * 16 + kk;
+    const uint32_t cx = (nx & 15u) == 0 ? (bx & ~15u) | ((bx & 7u) << 1) | ((bx >> 3) & 1u) : bx;
+    const uint32_t c0 = (g0 + cx) * 16, kg = by, k = kg * 16 + kk;
     uint32_t x[ROWS];
     if (k < r) {
         const uint32_t p = consts[k].p;
@@ -627,6 +667,16 @@ __global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t*
             *reinterpret_cast<v4u*>(tiles + ((size_t(i >> 8) * ksteps + kg) * 256 + (i & 255)) * 64 + q * 16) = v;
         }
     }
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void ntt_columns_inverse_tiles(const uint32_t* __restrict__ plane,
+                                                                 const PrimeConst* __restrict__ consts,
+                                                                 const uint2* __restrict__ tw, uint32_t ps,
+                                                                 uint32_t n, uint32_t n2, uint32_t C, uint32_t r,
+                                                                 uint32_t ksteps, int8_t* __restrict__ tiles,
+                                                                 uint32_t g0) {
+    inverse_tiles_body<ROWS>(plane, consts, tw, ps, n, n2, C, r, ksteps, tiles, g0, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // Row stages for C = 2048: one block per (row, prime), 256 threads x 8 values
@@ -712,14 +762,13 @@ __device__ __forceinline__ void row_exchange(uint32_t* buf, uint32_t (&x)[8], in
 }
 
 template <uint32_t N1>
-__global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ planes,
-                                                     const uint32_t* __restrict__ bplanes,
-                                                     const PrimeConst* __restrict__ consts,
-                                                     const uint2* __restrict__ twf, const uint2* __restrict__ twi,
-                                                     uint32_t ps, uint32_t n2, uint32_t rows, uint32_t r) {
+__device__ __forceinline__ void rows_2048_body(uint32_t* __restrict__ planes, const uint32_t* __restrict__ bplanes,
+                                               const PrimeConst* __restrict__ consts, const uint2* __restrict__ twf,
+                                               const uint2* __restrict__ twi, uint32_t ps, uint32_t n2, uint32_t rows,
+                                               uint32_t row, uint32_t k) {
     constexpr uint32_t n1 = N1;
     __shared__ uint32_t buf[2][kRowPadded];
-    const uint32_t row = blockIdx.x, k = blockIdx.y, t = threadIdx.x;
+    const uint32_t t = threadIdx.x;
     const uint32_t p = consts[k].p, pinv = consts[k].pinv;
     uint32_t* pa = planes + size_t(k) * ps + size_t(row) * kRowC;
     const uint32_t* pb = bplanes + size_t(k) * ps + size_t(row) * kRowC;
@@ -806,6 +855,15 @@ __global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ plan
     for (int e = 0; e < 8; ++e) pa[t + 256 * e] = xc[e];
 }
 
+template <uint32_t N1>
+__global__ __launch_bounds__(256) void ntt_rows_2048(uint32_t* __restrict__ planes,
+                                                     const uint32_t* __restrict__ bplanes,
+                                                     const PrimeConst* __restrict__ consts,
+                                                     const uint2* __restrict__ twf, const uint2* __restrict__ twi,
+                                                     uint32_t ps, uint32_t n2, uint32_t rows, uint32_t r) {
+    rows_2048_body<N1>(planes, bplanes, consts, twf, twi, ps, n2, rows, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // 6. reconstruction: one lane per coefficient, for q of up to 62 limbs (T of at
 // most 64 words), words in [j][n_pad] layout.  Output row k*256 + u is
@@ -818,16 +876,15 @@ struct ReconstructArgs {
     uint64_t mu;
 };
 
-__global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ words,
-                                                  uint32_t* __restrict__ tsc, uint32_t* __restrict__ c,
-                                                  const uint32_t* __restrict__ q,
-                                                  const uint32_t* __restrict__ neg_p,
-                                                  const uint32_t* __restrict__ corr_t,
-                                                  const uint32_t* __restrict__ corr_e,
-                                                  ReconstructArgs args) {
+__device__ __forceinline__ void reconstruct_body(const int64_t* __restrict__ words, uint32_t* __restrict__ tsc,
+                                                 uint32_t* __restrict__ c, const uint32_t* __restrict__ q,
+                                                 const uint32_t* __restrict__ neg_p,
+                                                 const uint32_t* __restrict__ corr_t,
+                                                 const uint32_t* __restrict__ corr_e, ReconstructArgs args,
+                                                 uint32_t bx) {
     __shared__ uint32_t tile[64][65];
     // Output rows k0 .. k0+63 lie in one tile of the set: coefficients i0 .. i0+63.
-    const uint32_t lane = threadIdx.x, k0 = blockIdx.x * 64;
+    const uint32_t lane = threadIdx.x, k0 = bx * 64;
     const uint32_t i0 = tile_at(args.ts, k0 / 256) * 256 + k0 % 256, i = i0 + lane;
     const uint32_t ps = args.n_pad, L = args.limbs, tw = args.t_words;
     const bool valid = i < args.n;
@@ -953,6 +1010,16 @@ __global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ wo
     }
 }
 
+__global__ __launch_bounds__(64) void reconstruct(const int64_t* __restrict__ words,
+                                                  uint32_t* __restrict__ tsc, uint32_t* __restrict__ c,
+                                                  const uint32_t* __restrict__ q,
+                                                  const uint32_t* __restrict__ neg_p,
+                                                  const uint32_t* __restrict__ corr_t,
+                                                  const uint32_t* __restrict__ corr_e,
+                                                  ReconstructArgs args) {
+    reconstruct_body(words, tsc, c, q, neg_p, corr_t, corr_e, args, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // 6'. reconstruction, one wave per coefficient (q wider than 62 limbs): the
 // coefficient's CRT word sums are coefficient-major (words[i][0 .. L+3]) and
@@ -1015,14 +1082,14 @@ __device__ __forceinline__ uint32_t normalize_round(int64_t x, int64_t& carry, u
     return w;
 }
 
-__global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restrict__ words, uint32_t* __restrict__ c,
-                                                         const uint32_t* __restrict__ q,
-                                                         const uint32_t* __restrict__ neg_p,
-                                                         const uint32_t* __restrict__ corr_t,
-                                                         const uint32_t* __restrict__ corr_e,
-                                                         WaveReconstructArgs args) {
+__device__ __forceinline__ void reconstruct_waves_body(const int64_t* __restrict__ words, uint32_t* __restrict__ c,
+                                                       const uint32_t* __restrict__ q,
+                                                       const uint32_t* __restrict__ neg_p,
+                                                       const uint32_t* __restrict__ corr_t,
+                                                       const uint32_t* __restrict__ corr_e,
+                                                       WaveReconstructArgs args, uint32_t bx) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t k = blockIdx.x * 4 + (threadIdx.x >> 6);  // output row
+    const uint32_t k = bx * 4 + (threadIdx.x >> 6);  // output row
     if (k >= args.ts.count * 256) return;  // whole waves
     const uint32_t i = tile_at(args.ts, k / 256) * 256 + k % 256;
     if (i >= args.n) return;
@@ -1119,6 +1186,119 @@ __global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restri
             break;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void reconstruct_waves(const int64_t* __restrict__ words, uint32_t* __restrict__ c,
+                                                         const uint32_t* __restrict__ q,
+                                                         const uint32_t* __restrict__ neg_p,
+                                                         const uint32_t* __restrict__ corr_t,
+                                                         const uint32_t* __restrict__ corr_e,
+                                                         WaveReconstructArgs args) {
+    reconstruct_waves_body(words, c, q, neg_p, corr_t, corr_e, args, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// 6''. Grouped launches: one stage of many products in one launch.  Entry j of
+// a launch owns blocks [start[j], start[j+1]) and runs them as the blocks of
+// product item[j]'s own launch of that stage (kernels.hpp, GroupItem).
+// ---------------------------------------------------------------------------
+struct GroupLaunch {
+    const GroupItem* items;
+    const uint32_t* start;  // count + 1 block offsets
+    const uint32_t* item;   // the product of each entry
+    uint32_t count;
+};
+
+// The product owning this block (a binary search over the offsets, the same
+// for every thread of the block) and the block's index in that product's grid.
+__device__ __forceinline__ const GroupItem& group_entry(const GroupLaunch& g, uint32_t& local) {
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = g.count;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (g.start[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    local = b - g.start[lo];
+    return g.items[g.item[lo]];
+}
+
+__device__ __forceinline__ TileSet every_tile(const GroupItem& it) { return tile_range(0, it.n_pad / kTile); }
+
+__global__ __launch_bounds__(256) void pack_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    pack_body(it.a, it.b, reinterpret_cast<uint32_t*>(it.packed), it.n, it.limbs, it.conv_k / kTK, it.n_pad / kTile,
+              every_tile(it), 0, 2, local);
+}
+
+__global__ __launch_bounds__(512) void conversion_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    conversion_body(it.wconv, it.packed, it.consts, it.planes, it.n, it.n_pad, it.r, it.r_pad, it.conv_rows,
+                    it.conv_k / kTK, every_tile(it), 0, 2, local);
+}
+
+// grid per product: (row_width / 256, r, 2)
+template <int ROWS>
+__global__ __launch_bounds__(256) void forward_reg_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    const uint32_t nx = it.row_width / 256, bx = local % nx, rest = local / nx;
+    forward_reg_body<ROWS>(it.planes, it.consts, it.tw_fwd, it.n_pad, it.n2, it.row_width, it.r_pad, 0, 0, bx,
+                           rest % it.r, rest / it.r);
+}
+
+// grid per product: (rows, r)
+template <uint32_t N1>
+__global__ __launch_bounds__(256) void rows_2048_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    rows_2048_body<N1>(it.planes, it.planes + size_t(it.r_pad) * it.n_pad, it.consts, it.tw_fwd, it.tw_inv, it.n_pad,
+                       it.n2, it.rows, local % it.rows, local / it.rows);
+}
+
+__global__ __launch_bounds__(256) void rows_product_group(GroupLaunch g) {
+    extern __shared__ uint32_t lds32[];
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    rows_product_body(it.planes, it.planes + size_t(it.r_pad) * it.n_pad, it.consts, it.tw_fwd, it.tw_inv, it.n_pad,
+                      it.n1, it.n2, it.row_width, it.rows, lds32, local % it.rows, local / it.rows);
+}
+
+// grid per product: (row_width / 16, r_pad / 16)
+template <int ROWS>
+__global__ __launch_bounds__(256) void inverse_tiles_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    const uint32_t nx = it.row_width / 16;
+    inverse_tiles_body<ROWS>(it.planes, it.consts, it.tw_inv, it.n_pad, it.n, it.n2, it.row_width, it.r,
+                             it.r_pad / 16, it.rtiles, 0, local % nx, local / nx, nx);
+}
+
+template <bool kByCoefficient>
+__global__ __launch_bounds__(512) void crt_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    crt_body<kByCoefficient>(it.wcrt, it.rtiles, it.words, it.n, it.n_pad, it.crt_rows, it.crt_k / kTK, it.crt_words,
+                             it.ws, every_tile(it), local);
+}
+
+__global__ __launch_bounds__(64) void reconstruct_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    const ReconstructArgs args{it.n, it.n_pad, it.limbs, it.t_words, every_tile(it), it.s_shift, it.k_shift, it.mu};
+    reconstruct_body(it.words, it.t_scratch, it.c, it.q, it.neg_p, it.corr_t, it.corr_e, args, local);
+}
+
+__global__ __launch_bounds__(256) void reconstruct_waves_group(GroupLaunch g) {
+    uint32_t local;
+    const GroupItem& it = group_entry(g, local);
+    const WaveReconstructArgs args{it.n, it.ws, it.limbs, it.t_words, every_tile(it),
+                                   it.top_m, it.top_e, it.k_shift, it.mu};
+    reconstruct_waves_body(it.words, it.c, it.q, it.neg_p, it.corr_t, it.corr_e, args, local);
 }
 
 __global__ void fill_zero(uint32_t* __restrict__ c, size_t words) {
@@ -1480,6 +1660,163 @@ void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* wo
     hipLaunchKernelGGL(reconstruct, dim3(ts.count * 4), dim3(64), 0, stream, words, t_scratch, c, t.q, t.neg_p,
                        t.corr_t, t.corr_e, args);
     check_launch("reconstruct");
+}
+
+GroupItem group_item(const Shape& s, const DeviceTables& t, const DeviceWork& w, const uint32_t* a,
+                     const uint32_t* b, uint32_t* c) {
+    GroupItem it{};
+    it.n = s.n, it.n_pad = s.n_pad, it.limbs = s.limbs, it.r = s.r, it.r_pad = s.r_pad;
+    it.conv_rows = s.conv_rows, it.conv_k = s.conv_k, it.crt_rows = s.crt_rows, it.crt_k = s.crt_k;
+    it.crt_words = s.crt_words, it.ws = word_stride(s), it.t_words = s.t_words;
+    it.n1 = s.n1, it.n2 = s.n2, it.rows = s.rows, it.row_width = s.row_width;
+    it.s_shift = s.s_shift, it.k_shift = s.k_shift;
+    it.top_m = s.s_shift >= 64 ? s.s_shift / 32 - 1 : 0;  // as launch_reconstruct
+    it.top_e = s.s_shift - 32 * it.top_m;
+    it.mu = s.mu;
+    it.consts = t.consts, it.wconv = t.wconv, it.tw_fwd = t.tw_fwd, it.tw_inv = t.tw_inv, it.wcrt = t.wcrt;
+    it.q = t.q, it.neg_p = t.neg_p, it.corr_t = t.corr_t, it.corr_e = t.corr_e;
+    it.a = a, it.b = b, it.c = c;
+    it.packed = w.packed, it.planes = w.planes, it.rtiles = w.rtiles, it.words = w.words, it.t_scratch = w.t_words;
+    return it;
+}
+
+bool groupable(const Shape& s) {
+    if (s.trivial) return false;
+    if (s.rows >= 2 && (s.row_width % 256 != 0 || s.rows > 64)) return false;  // forward columns through LDS
+    if (s.row_width % 16 != 0 || s.rows > 64) return false;                      // inverse columns through LDS
+    return true;
+}
+
+namespace {
+
+// Launches of one batch: pack, conversion, forward columns (one per row count),
+// rows (one per block size n1, and the LDS kernel), inverse columns (one per
+// row count), CRT GEMM (two word layouts), reconstruction (two kernels).
+constexpr size_t kGroupLaunches = 2 + 6 + 5 + 7 + 2 + 2;
+
+enum GroupKind { kPack, kConversion, kForward, kRows2048, kRowsProduct, kInverse, kCrt, kRecLane, kRecWave };
+
+}  // namespace
+
+size_t group_table_words(size_t count) { return kGroupLaunches * (2 * count + 1); }
+
+void launch_group(const std::vector<GroupItem>& items, const GroupItem* dev_items, uint32_t* dev_tables,
+                  hipStream_t stream) {
+    const uint32_t count = uint32_t(items.size());
+    if (!count) return;
+    struct Launch {
+        GroupKind kind;
+        uint32_t variant;
+        uint32_t offset, entries;
+        uint64_t blocks;
+        size_t lds;
+    };
+    std::vector<uint32_t> tables(group_table_words(count));
+    std::vector<Launch> launches;
+    size_t used = 0;
+    // One launch over the items for which blocks(it) > 0.
+    auto add = [&](GroupKind kind, uint32_t variant, auto blocks) {
+        std::vector<uint32_t> sel;
+        for (uint32_t i = 0; i < count; ++i)
+            if (blocks(items[i]) > 0) sel.push_back(i);
+        if (sel.empty()) return;
+        if (launches.size() == kGroupLaunches) throw std::logic_error("launch_group: more launches than planned");
+        uint32_t* start = tables.data() + used;
+        uint32_t* item = start + sel.size() + 1;
+        uint64_t total = 0;
+        size_t lds = 0;
+        for (size_t j = 0; j < sel.size(); ++j) {
+            start[j] = uint32_t(total);
+            item[j] = sel[j];
+            total += blocks(items[sel[j]]);
+            lds = std::max<size_t>(lds, size_t(2) * items[sel[j]].row_width * 4);
+        }
+        if (total >= (uint64_t(1) << 31)) throw std::invalid_argument("launch_group: batch too large for one grid");
+        start[sel.size()] = uint32_t(total);
+        launches.push_back(Launch{kind, variant, uint32_t(used), uint32_t(sel.size()), total, lds});
+        used += 2 * sel.size() + 1;
+    };
+    auto ntiles = [](const GroupItem& it) { return uint64_t(it.n_pad / kTile); };
+    auto rows_2048 = [](const GroupItem& it) { return it.row_width == kRowC && it.n1 <= 8; };
+    add(kPack, 0, [&](const GroupItem& it) { return 128 * ntiles(it); });
+    add(kConversion, 0, [&](const GroupItem& it) { return uint64_t(it.conv_rows / kTile) * 2 * ntiles(it); });
+    for (uint32_t R = 2; R <= 64; R *= 2)
+        add(kForward, R, [&](const GroupItem& it) {
+            return it.rows == R ? uint64_t(it.row_width / 256) * it.r * 2 : 0;
+        });
+    for (uint32_t b = 1; b <= 8; b *= 2)
+        add(kRows2048, b, [&](const GroupItem& it) {
+            return rows_2048(it) && it.n1 == b ? uint64_t(it.rows) * it.r : 0;
+        });
+    add(kRowsProduct, 0, [&](const GroupItem& it) { return rows_2048(it) ? 0 : uint64_t(it.rows) * it.r; });
+    for (uint32_t R = 1; R <= 64; R *= 2)
+        add(kInverse, R, [&](const GroupItem& it) {
+            return it.rows == R ? uint64_t(it.row_width / 16) * (it.r_pad / 16) : 0;
+        });
+    add(kCrt, 0, [&](const GroupItem& it) { return it.t_words > 64 ? 0 : uint64_t(it.crt_rows / kTile) * ntiles(it); });
+    add(kCrt, 1, [&](const GroupItem& it) { return it.t_words > 64 ? uint64_t(it.crt_rows / kTile) * ntiles(it) : 0; });
+    add(kRecLane, 0, [&](const GroupItem& it) { return it.t_words > 64 ? 0 : 4 * ntiles(it); });
+    add(kRecWave, 0, [&](const GroupItem& it) { return it.t_words > 64 ? 64 * ntiles(it) : 0; });
+    const hipError_t copied = hipMemcpy(dev_tables, tables.data(), used * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (copied != hipSuccess) throw std::runtime_error(std::string("launch_group tables: ") + hipGetErrorString(copied));
+    for (const Launch& L : launches) {
+        const GroupLaunch g{dev_items, dev_tables + L.offset, dev_tables + L.offset + L.entries + 1, L.entries};
+        const dim3 grid(uint32_t(L.blocks));
+        switch (L.kind) {
+            case kPack:
+                hipLaunchKernelGGL(pack_group, grid, dim3(256), 0, stream, g);
+                break;
+            case kConversion:
+                hipLaunchKernelGGL(conversion_group, grid, dim3(kGemmThreads), 0, stream, g);
+                break;
+            case kForward:
+                switch (L.variant) {
+#define PM_GFWD(R)                                                                      \
+    case R:                                                                             \
+        hipLaunchKernelGGL(forward_reg_group<R>, grid, dim3(256), 0, stream, g);         \
+        break;
+                    PM_GFWD(2) PM_GFWD(4) PM_GFWD(8) PM_GFWD(16) PM_GFWD(32) PM_GFWD(64)
+#undef PM_GFWD
+                }
+                break;
+            case kRows2048:
+                switch (L.variant) {
+#define PM_GROWS(B)                                                                     \
+    case B:                                                                             \
+        hipLaunchKernelGGL(rows_2048_group<B>, grid, dim3(256), 0, stream, g);           \
+        break;
+                    PM_GROWS(1) PM_GROWS(2) PM_GROWS(4) PM_GROWS(8)
+#undef PM_GROWS
+                }
+                break;
+            case kRowsProduct:
+                hipLaunchKernelGGL(rows_product_group, grid, dim3(256), L.lds, stream, g);
+                break;
+            case kInverse:
+                switch (L.variant) {
+#define PM_GINV(R)                                                                      \
+    case R:                                                                             \
+        hipLaunchKernelGGL(inverse_tiles_group<R>, grid, dim3(256), 0, stream, g);       \
+        break;
+                    PM_GINV(1) PM_GINV(2) PM_GINV(4) PM_GINV(8) PM_GINV(16) PM_GINV(32) PM_GINV(64)
+#undef PM_GINV
+                }
+                break;
+            case kCrt:
+                if (L.variant)
+                    hipLaunchKernelGGL(crt_group<true>, grid, dim3(kGemmThreads), 0, stream, g);
+                else
+                    hipLaunchKernelGGL(crt_group<false>, grid, dim3(kGemmThreads), 0, stream, g);
+                break;
+            case kRecLane:
+                hipLaunchKernelGGL(reconstruct_group, grid, dim3(64), 0, stream, g);
+                break;
+            case kRecWave:
+                hipLaunchKernelGGL(reconstruct_waves_group, grid, dim3(256), 0, stream, g);
+                break;
+        }
+        check_launch("grouped stage");
+    }
 }
 
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream) {

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "chunks.hpp"
 #include "plan.hpp"
@@ -101,6 +102,47 @@ inline bool reconstruct_writes_once(const Shape& s) { return s.t_words <= 64; }
 void launch_reconstruct(const Shape& s, const DeviceTables& t, const int64_t* words, uint32_t* t_scratch,
                         uint32_t* c, const TileSet& ts, hipStream_t stream);
 void launch_zero(uint32_t* c, size_t words, hipStream_t stream);
+
+// Grouped launches: every stage of a batch of device products (each its own
+// plan: shape, q, tables and working buffers) as one launch per stage and
+// kernel variant, instead of one launch per stage and product.  A launch walks
+// a work list: entry j owns blocks [start[j], start[j+1]) of the grid and runs
+// them for product item[j] with that product's parameters.
+struct GroupItem {
+    uint32_t n, n_pad, limbs, r, r_pad, conv_rows, conv_k, crt_rows, crt_k, crt_words, ws, t_words;
+    uint32_t n1, n2, rows, row_width, s_shift, k_shift, top_m, top_e;
+    uint64_t mu;
+    const PrimeConst* consts;
+    const int8_t* wconv;
+    const uint2* tw_fwd;
+    const uint2* tw_inv;
+    const int8_t* wcrt;
+    const uint32_t* q;
+    const uint32_t* neg_p;
+    const uint32_t* corr_t;
+    const uint32_t* corr_e;
+    const uint32_t* a;
+    const uint32_t* b;
+    uint32_t* c;
+    int8_t* packed;
+    uint32_t* planes;
+    int8_t* rtiles;
+    int64_t* words;
+    uint32_t* t_scratch;
+};
+// The item of one plain product (a, b -> c on the working buffers w).
+GroupItem group_item(const Shape& s, const DeviceTables& t, const DeviceWork& w, const uint32_t* a,
+                     const uint32_t* b, uint32_t* c);
+// Whether every stage of the shape has a grouped kernel (else the product runs
+// on its own): q > 1, and column stages held in registers.
+bool groupable(const Shape& s);
+// Words of device memory the launch tables of `count` products need.
+size_t group_table_words(size_t count);
+// Runs the stages of every item, in order, on `stream`: `items` in host memory
+// decide the launches, `dev_items` is their copy in device memory and
+// `dev_tables` (group_table_words(items.size()) words) receives the work lists.
+void launch_group(const std::vector<GroupItem>& items, const GroupItem* dev_items, uint32_t* dev_tables,
+                  hipStream_t stream);
 
 // Plan-time table construction on the device (plans built with Tables::kDevice):
 // beta_k, the merged-twist twiddles, both conversion weight matrices and their

@@ -133,6 +133,7 @@ def load_library(path=None):
         "pm_multiply": ([vp, vp, vp, vp], ctypes.c_int),
         "pm_multiply_device": ([vp, vp, vp, vp, vp], ctypes.c_int),
         "pm_multiply_device_scratch": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "pm_multiply_device_batch": ([vp, vp, vp, vp, ctypes.c_size_t, vp], ctypes.c_int),
         "pm_multiply_begin": ([vp, vp, vp], ctypes.c_int),
         "pm_multiply_finish": ([vp, vp], ctypes.c_int),
         "pm_last_error": ([], ctypes.c_char_p),
@@ -375,6 +376,20 @@ def multiply_batch(products, device_budget_bytes, lanes=4):
     statuses = (ctypes.c_int * len(built))()
     _check(lib, lib.pm_multiply_batch(reqs, len(built), int(device_budget_bytes), int(lanes), statuses))
     return [b[1] for b in built]
+
+
+def multiply_device_batch(plans, a_ptrs, b_ptrs, c_ptrs, stream=None):
+    """Many device-resident products, plans[i]: a[i] * b[i] -> c[i] (device
+    pointers as ints), each stage one launch for the whole batch
+    (pm_multiply_device_batch)."""
+    lib = load_library()
+    count = len(plans)
+    if not (len(a_ptrs) == len(b_ptrs) == len(c_ptrs) == count):
+        raise ValueError("one a, b and c pointer per plan")
+    handles = (ctypes.c_void_p * count)(*[p.handle.value for p in plans])
+    arrays = [(ctypes.c_void_p * count)(*[int(x) for x in xs]) for xs in (a_ptrs, b_ptrs, c_ptrs)]
+    _check(lib, lib.pm_multiply_device_batch(handles, arrays[0], arrays[1], arrays[2], count,
+                                             ctypes.c_void_p(stream) if stream else None))
 
 
 class Server:

@@ -99,6 +99,23 @@ pm_status pm_multiply_finish(pm_plan* plan, uint32_t* c);
 pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c,
                              void* stream);
 
+/* Many device-resident products, each on its own plan (its own N, q and
+ * tables): plans[i] multiplies a[i] * b[i] into c[i] (device pointers to
+ * [N][L] words on the plans' device).  No counterpart in the reference (it
+ * multiplies one pair per call); it is the batched form for callers with many
+ * products in HBM at once.  Each stage runs as one launch for every product
+ * of the batch (per kernel variant), a work list mapping the launch's blocks
+ * to (product, block of that product's own launch), instead of one launch per
+ * stage and product: small products, which cannot fill the GPU alone and are
+ * bound by launches, then share every launch.  A product whose shape has a
+ * stage without a grouped kernel (q == 1, or more than 64 transform rows)
+ * runs on its own, on the same stream.  Every plan must be idle and appear
+ * once (its working buffers are per product); all on one device.  stream is
+ * a hipStream_t (NULL = plans[0]'s own stream).  Synchronizes the stream
+ * before returning.  count == 0 does nothing. */
+pm_status pm_multiply_device_batch(pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                                   uint32_t* const* c, size_t count, void* stream);
+
 /* pm_multiply_device without any allocation: no counterpart in the reference.
  * It is for callers that cannot allocate device memory in the call (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -75,6 +75,11 @@ class DeviceMemory:
         self._check(self.rt.hipMemcpy(ptr, array.ctypes.data, array.nbytes, 1), "hipMemcpy H2D")
         return ptr
 
+    def write(self, ptr, array):
+        """Copies `array` into the existing buffer at `ptr`."""
+        array = np.ascontiguousarray(array)
+        self._check(self.rt.hipMemcpy(ptr, array.ctypes.data, array.nbytes, 1), "hipMemcpy H2D")
+
     def download(self, ptr, shape, dtype):
         out = np.empty(shape, dtype=dtype)
         self._check(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")

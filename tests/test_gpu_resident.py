@@ -109,3 +109,54 @@ def test_begun_product_blocks_the_plans_other_entries(polymul, workloads, device
         finally:
             for ptr in (da, db, dc):
                 device_memory.free(ptr)
+
+
+def test_grouped_batch_matches_device_entry(polymul, workloads, oracle, device_memory):
+    """pm_multiply_device_batch over a batch of mixed shapes -- the reference
+    point's N = 65536 with a 1747-bit q, B-mid (N = 32768, split transform),
+    q wider than 62 limbs (wave reconstruction), N = 1024 and 2048 (no column
+    stage), and two products that run on their own inside the batch (q = 1,
+    N = 8) -- gives every product's pm_multiply_device result; a plan named
+    twice, a null operand and plans of an empty batch are handled."""
+    print("[gpu-case] grouped batch, mixed shapes", flush=True)
+    rng = np.random.default_rng(4242)
+    cases = []
+    n, width, limbs, q, a, b = workloads.seeded_case("mid")
+    cases.append((n, limbs, q, a, b))
+    for n, limbs in ((65536, 55), (4096, 80), (1024, 8), (2048, 20), (8, 3)):
+        q = workloads.random_modulus(limbs, rng)
+        cases.append((n, limbs, q, workloads.uniform_below(q, n, rng), workloads.uniform_below(q, n, rng)))
+    one = np.zeros(2, dtype=np.uint32)
+    one[0] = 1
+    cases.append((64, 2, one, np.zeros((64, 2), dtype=np.uint32), np.zeros((64, 2), dtype=np.uint32)))
+    plans = [polymul.Plan(n, 32 * limbs, limbs, q) for n, limbs, q, _a, _b in cases]
+    bufs = [(device_memory.upload(a), device_memory.upload(b), device_memory.alloc(a.nbytes))
+            for _n, _l, _q, a, b in cases]
+    try:
+        expect = []
+        for plan, (pa, pb, pc), (n, limbs, *_r) in zip(plans, bufs, cases):
+            plan.multiply_device(pa, pb, pc)
+            expect.append(device_memory.download(pc, (n, limbs), np.uint32))
+        n, limbs, q, a, b = cases[4]  # N = 1024: against the oracle too
+        assert np.array_equal(expect[4], oracle.kronecker_gmp(a, b, q))
+        for attempt in range(2):
+            for (n, limbs, *_r), (_pa, _pb, pc) in zip(cases, bufs):
+                device_memory.write(pc, np.full((n, limbs), 0xDEADBEEF, dtype=np.uint32))
+            polymul.multiply_device_batch(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+            for i, ((n, limbs, *_r), (_pa, _pb, pc)) in enumerate(zip(cases, bufs)):
+                got = device_memory.download(pc, (n, limbs), np.uint32)
+                assert np.array_equal(got, expect[i]), "product %d (N=%d, L=%d), pass %d" % (i, n, limbs, attempt)
+        with pytest.raises(polymul.PolymulError) as err:
+            polymul.multiply_device_batch(plans[:2] + plans[:1], [bufs[0][0], bufs[1][0], bufs[0][0]],
+                                          [bufs[0][1], bufs[1][1], bufs[0][1]], [bufs[0][2], bufs[1][2], bufs[0][2]])
+        assert err.value.status == 1
+        with pytest.raises(polymul.PolymulError) as err:
+            polymul.multiply_device_batch(plans[:1], [0], [bufs[0][1]], [bufs[0][2]])
+        assert err.value.status == 1
+        polymul.multiply_device_batch([], [], [], [])
+    finally:
+        for buffers in bufs:
+            for ptr in buffers:
+                device_memory.free(ptr)
+        for plan in plans:
+            plan.close()

@@ -9,8 +9,8 @@ tests/golden/workload_digests.json.
   batched          the 64 products of the batched workload through
                    pm_multiply_batch (60 GB budget, 4 lanes)
   small batched    64 small products (N <= 8192, L <= 128) through
-                   pm_multiply_batch and through pm_multiply_device from
-                   four threads at once
+                   pm_multiply_batch, through pm_multiply_device from four
+                   threads at once and through pm_multiply_device_batch
   resident         64 random a times one fixed b, b's residues on the device
   served           the 48 requests of the served workload with Poisson
                    arrivals, 200 GB budget: each result in the mix must be the
@@ -83,8 +83,9 @@ def test_batched_workload(polymul, workloads):
 
 def test_small_batched_workload(polymul, workloads, device_memory):
     """The mix of 64 small products (N <= 8192, L <= 128) through
-    pm_multiply_batch, and through pm_multiply_device from four threads at
-    once (each plan on its own stream), against the oracle's digests."""
+    pm_multiply_batch, through pm_multiply_device from four threads at once
+    (each plan on its own stream), and through pm_multiply_device_batch (each
+    stage one launch for all 64), against the oracle's digests."""
     import threading
 
     g = GOLDEN["batch_small"]["products"]
@@ -124,6 +125,14 @@ def test_small_batched_workload(polymul, workloads, device_memory):
         bad = [i for i, ((n, limbs, *_r), (_a, _b, pc), e) in enumerate(zip(products, bufs, g))
                if workloads.digest(device_memory.download(pc, (n, limbs), np.uint32)) != e["c"]]
         assert not bad, "device products from four threads differ from the oracle: %s" % bad
+        # the grouped entry: each stage one launch for all 64 products, twice in a row
+        for attempt in range(2):
+            for (n, limbs, *_r), (_a, _b, pc) in zip(products, bufs):
+                device_memory.write(pc, np.zeros((n, limbs), dtype=np.uint32))
+            polymul.multiply_device_batch(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+            bad = [i for i, ((n, limbs, *_r), (_a, _b, pc), e) in enumerate(zip(products, bufs, g))
+                   if workloads.digest(device_memory.download(pc, (n, limbs), np.uint32)) != e["c"]]
+            assert not bad, "grouped products (pass %d) differ from the oracle: %s" % (attempt, bad)
     finally:
         for pa, pb, pc in bufs:
             for ptr in (pa, pb, pc):
