@@ -7,6 +7,9 @@
 #include "polymul.h"
 
 #include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
 
 #include "state.hpp"
 
@@ -96,6 +99,27 @@ pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b
     });
 }
 
+namespace {
+
+// Per device: the buffer of pm_multiply_device_batch's items and work lists,
+// kept for the process (a few tens of KB per hundred products).
+struct BatchBuffer {
+    std::mutex mutex;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+BatchBuffer& batch_buffer(int device) {
+    static std::mutex guard;
+    static std::map<int, std::unique_ptr<BatchBuffer>> buffers;
+    std::lock_guard<std::mutex> lock(guard);
+    std::unique_ptr<BatchBuffer>& b = buffers[device];
+    if (!b) b = std::make_unique<BatchBuffer>();
+    return *b;
+}
+
+}  // namespace
+
 pm_status pm_multiply_device_batch(pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                                    uint32_t* const* c, size_t count, void* stream) {
     return guarded([&] {
@@ -127,22 +151,23 @@ pm_status pm_multiply_device_batch(pm_plan* const* plans, const uint32_t* const*
                 plans[i]->run(a[i], b[i], c[i], PM_STAGE_OUTPUT, st);  // on its own, ahead of the grouped launches
         }
         if (!items.empty()) {
-            // the items and the launches' work lists, in one buffer for the call
-            const size_t item_bytes = (items.size() * sizeof(pm::GroupItem) + 255) / 256 * 256;
-            const size_t table_bytes = pm::group_table_words(items.size()) * sizeof(uint32_t);
-            void* mem = nullptr;
-            hip_check(hipMalloc(&mem, item_bytes + table_bytes), "hipMalloc");
-            struct Release {
-                void* p;
-                ~Release() { (void)hipFree(p); }
-            } release{mem};
+            // The items and the launches' work lists go into the device's batch
+            // buffer, held (and grown) across calls; a second batch on the same
+            // device waits here until the first has finished with it.
+            BatchBuffer& buf = batch_buffer(device);
+            std::lock_guard<std::mutex> hold(buf.mutex);
+            const size_t need = pm::group_bytes(items.size());
+            if (buf.bytes < need) {
+                if (buf.ptr) hip_check(hipFree(buf.ptr), "hipFree");
+                buf.ptr = nullptr;
+                buf.bytes = 0;
+                hip_check(hipMalloc(&buf.ptr, need), "hipMalloc");
+                buf.bytes = need;
+            }
             try {
-                hip_check(hipMemcpy(mem, items.data(), items.size() * sizeof(pm::GroupItem), hipMemcpyHostToDevice),
-                          "upload batch");
-                pm::launch_group(items, static_cast<const pm::GroupItem*>(mem),
-                                 reinterpret_cast<uint32_t*>(static_cast<char*>(mem) + item_bytes), st);
+                pm::launch_group(items, buf.ptr, st);
             } catch (...) {
-                (void)hipStreamSynchronize(st);  // nothing queued may outlive the buffer
+                (void)hipStreamSynchronize(st);  // nothing queued may outlive the hold on the buffer
                 throw;
             }
             hip_check(hipStreamSynchronize(st), "multiply batch");

@@ -29,6 +29,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -1696,14 +1697,19 @@ constexpr size_t kGroupLaunches = 2 + 6 + 5 + 7 + 2 + 2;
 
 enum GroupKind { kPack, kConversion, kForward, kRows2048, kRowsProduct, kInverse, kCrt, kRecLane, kRecWave };
 
-}  // namespace
-
+size_t group_item_bytes(size_t count) { return (count * sizeof(GroupItem) + 255) / 256 * 256; }
 size_t group_table_words(size_t count) { return kGroupLaunches * (2 * count + 1); }
 
-void launch_group(const std::vector<GroupItem>& items, const GroupItem* dev_items, uint32_t* dev_tables,
-                  hipStream_t stream) {
+}  // namespace
+
+size_t group_bytes(size_t count) { return group_item_bytes(count) + group_table_words(count) * sizeof(uint32_t); }
+
+void launch_group(const std::vector<GroupItem>& items, void* dev_buffer, hipStream_t stream) {
     const uint32_t count = uint32_t(items.size());
     if (!count) return;
+    const size_t item_bytes = group_item_bytes(count);
+    const GroupItem* dev_items = static_cast<const GroupItem*>(dev_buffer);
+    uint32_t* dev_tables = reinterpret_cast<uint32_t*>(static_cast<char*>(dev_buffer) + item_bytes);
     struct Launch {
         GroupKind kind;
         uint32_t variant;
@@ -1757,8 +1763,12 @@ void launch_group(const std::vector<GroupItem>& items, const GroupItem* dev_item
     add(kCrt, 1, [&](const GroupItem& it) { return it.t_words > 64 ? uint64_t(it.crt_rows / kTile) * ntiles(it) : 0; });
     add(kRecLane, 0, [&](const GroupItem& it) { return it.t_words > 64 ? 0 : 4 * ntiles(it); });
     add(kRecWave, 0, [&](const GroupItem& it) { return it.t_words > 64 ? 64 * ntiles(it) : 0; });
-    const hipError_t copied = hipMemcpy(dev_tables, tables.data(), used * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (copied != hipSuccess) throw std::runtime_error(std::string("launch_group tables: ") + hipGetErrorString(copied));
+    // the items and the work lists, one copy
+    std::vector<char> upload(item_bytes + used * sizeof(uint32_t));
+    std::memcpy(upload.data(), items.data(), count * sizeof(GroupItem));
+    std::memcpy(upload.data() + item_bytes, tables.data(), used * sizeof(uint32_t));
+    const hipError_t copied = hipMemcpy(dev_buffer, upload.data(), upload.size(), hipMemcpyHostToDevice);
+    if (copied != hipSuccess) throw std::runtime_error(std::string("launch_group upload: ") + hipGetErrorString(copied));
     for (const Launch& L : launches) {
         const GroupLaunch g{dev_items, dev_tables + L.offset, dev_tables + L.offset + L.entries + 1, L.entries};
         const dim3 grid(uint32_t(L.blocks));

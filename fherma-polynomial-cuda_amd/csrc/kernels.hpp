@@ -136,13 +136,13 @@ GroupItem group_item(const Shape& s, const DeviceTables& t, const DeviceWork& w,
 // Whether every stage of the shape has a grouped kernel (else the product runs
 // on its own): q > 1, and column stages held in registers.
 bool groupable(const Shape& s);
-// Words of device memory the launch tables of `count` products need.
-size_t group_table_words(size_t count);
-// Runs the stages of every item, in order, on `stream`: `items` in host memory
-// decide the launches, `dev_items` is their copy in device memory and
-// `dev_tables` (group_table_words(items.size()) words) receives the work lists.
-void launch_group(const std::vector<GroupItem>& items, const GroupItem* dev_items, uint32_t* dev_tables,
-                  hipStream_t stream);
+// Bytes of device memory a batch of `count` products needs: the items, then
+// the launches' work lists.
+size_t group_bytes(size_t count);
+// Runs the stages of every item, in order, on `stream`, after copying the items
+// and the work lists (one copy) into `dev_buffer` (group_bytes(items.size())
+// bytes, unused by any launch still queued).
+void launch_group(const std::vector<GroupItem>& items, void* dev_buffer, hipStream_t stream);
 
 // Plan-time table construction on the device (plans built with Tables::kDevice):
 // beta_k, the merged-twist twiddles, both conversion weight matrices and their

@@ -111,8 +111,11 @@ pm_status pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b
  * stage without a grouped kernel (q == 1, or more than 64 transform rows)
  * runs on its own, on the same stream.  Every plan must be idle and appear
  * once (its working buffers are per product); all on one device.  stream is
- * a hipStream_t (NULL = plans[0]'s own stream).  Synchronizes the stream
- * before returning.  count == 0 does nothing. */
+ * a hipStream_t (NULL = plans[0]'s own stream).  The items and work lists go
+ * into a device buffer the library keeps per device (grown as needed, about
+ * 0.5 KB per product, held until the process ends); batch calls on one device
+ * take turns with it.  Synchronizes the stream before returning.  count == 0
+ * does nothing. */
 pm_status pm_multiply_device_batch(pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                                    uint32_t* const* c, size_t count, void* stream);
 
