@@ -237,11 +237,17 @@ __device__ __forceinline__ void gemm_mainloop(const int8_t* __restrict__ a_panel
 // ---------------------------------------------------------------------------
 // 0. operand packing: [N][L] u32 -> [2][n_pad/256][conv_k/64][256][64] bytes,
 //    each byte XOR 0x80 (u -> u-128 as int8); padding words (beyond L, or rows
-//    beyond N) hold the value 0.  One wave per coefficient row: lane l moves
-//    words l, l+64, ... of the row, so every load reads 256 contiguous bytes and
-//    each 128-byte line of the input is consumed by one wave (a block holds 4
-//    consecutive rows, so only lines at block edges are shared).
+//    beyond N) hold the value 0.  One wave per coefficient row (two or four
+//    short rows per wave, below): lane l moves words l, l+64, ... of the row,
+//    so every load reads 256 contiguous bytes and each 128-byte line of the
+//    input is consumed by one wave (a block holds consecutive rows, so only
+//    lines at block edges are shared).
 // ---------------------------------------------------------------------------
+// Rows of one wave: packed rows of 16 or 32 words (one or two K steps, L of
+// at most 8 or 16 limbs) share a wave four or two at a time, so that short
+// rows do not leave most of its lanes idle.
+constexpr uint32_t pack_rows_per_wave(uint32_t ksteps) { return ksteps == 1 ? 4 : ksteps == 2 ? 2 : 1; }
+
 // Packs the tiles of `ts` of operands op0 .. op0+nops-1.  The k-th tile's rows
 // are read from rows k*256 .. k*256+255 of a (or b): the caller passes the
 // operands' rows in the set's order (all of [N][L] for the set of every tile).
@@ -249,9 +255,11 @@ __device__ __forceinline__ void pack_body(const uint32_t* __restrict__ a, const 
                                           uint32_t* __restrict__ packed, uint32_t n, uint32_t limbs,
                                           uint32_t ksteps, uint32_t ntiles, TileSet ts, uint32_t op0,
                                           uint32_t nops, uint32_t bx) {
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t rpw = pack_rows_per_wave(ksteps), span = 64 / rpw;  // lanes per row
+    const uint32_t lane = (threadIdx.x & 63) % span;
     const uint32_t per_op = ts.count * kTile;
-    const uint32_t gr = bx * 4 + (threadIdx.x >> 6);  // row over all operands of the launch
+    // row over all operands of the launch
+    const uint32_t gr = (bx * 4 + (threadIdx.x >> 6)) * rpw + (threadIdx.x & 63) / span;
     if (gr >= nops * per_op) return;
     const uint32_t op = op0 + gr / per_op, rr = gr % per_op;
     const uint32_t tn = tile_at(ts, rr / kTile), row = rr % kTile, coef = tn * kTile + row;
@@ -263,16 +271,16 @@ __device__ __forceinline__ void pack_body(const uint32_t* __restrict__ a, const 
     uint32_t* dst = packed + (size_t(op) * ntiles + tn) * ksteps * (kTileBytes / 4) + row * (kTK / 4);
     const uint32_t words = ksteps * (kTK / 4);
     constexpr uint32_t kPackLoads = 8;
-    for (uint32_t w0 = lane; w0 < words; w0 += 64 * kPackLoads) {
+    for (uint32_t w0 = lane; w0 < words; w0 += span * kPackLoads) {
         uint32_t v[kPackLoads];
 #pragma unroll
         for (uint32_t u = 0; u < kPackLoads; ++u) {
-            const uint32_t w = w0 + 64 * u;
+            const uint32_t w = w0 + span * u;
             v[u] = src[w < limbs ? w : limbs - 1];  // clamped into the row; padding words are zeroed below
         }
 #pragma unroll
         for (uint32_t u = 0; u < kPackLoads; ++u) {
-            const uint32_t w = w0 + 64 * u;
+            const uint32_t w = w0 + span * u;
             if (w < words)
                 dst[size_t(w / (kTK / 4)) * (kTileBytes / 4) + w % (kTK / 4)] =
                     ((live && w < limbs) ? v[u] : 0u) ^ 0x80808080u;
@@ -1535,8 +1543,9 @@ void launch_pack(const Shape& s, const uint32_t* a, const uint32_t* b, int8_t* p
                  uint32_t op0, uint32_t nops, hipStream_t stream) {
     check_tiles(s, ts, "pack_operands");
     const uint32_t ksteps = s.conv_k / kTK, ntiles = s.n_pad / kTile;
-    const size_t rows = size_t(nops) * ts.count * kTile;  // one wave per row, 4 rows per block
-    dim3 grid(uint32_t((rows + 3) / 4));
+    const size_t rows = size_t(nops) * ts.count * kTile;  // 4 waves per block, 1, 2 or 4 rows per wave
+    const uint32_t per_block = 4 * pack_rows_per_wave(ksteps);
+    dim3 grid(uint32_t((rows + per_block - 1) / per_block));
     hipLaunchKernelGGL(pack_operands, grid, dim3(256), 0, stream, a, b, reinterpret_cast<uint32_t*>(packed), s.n,
                        s.limbs, ksteps, ntiles, ts, op0, nops);
     check_launch("pack_operands");
@@ -1744,7 +1753,9 @@ void launch_group(const std::vector<GroupItem>& items, void* dev_buffer, hipStre
     };
     auto ntiles = [](const GroupItem& it) { return uint64_t(it.n_pad / kTile); };
     auto rows_2048 = [](const GroupItem& it) { return it.row_width == kRowC && it.n1 <= 8; };
-    add(kPack, 0, [&](const GroupItem& it) { return 128 * ntiles(it); });
+    add(kPack, 0, [&](const GroupItem& it) {  // as launch_pack: 2 * ntiles * 256 rows
+        return 128 * ntiles(it) / pack_rows_per_wave(it.conv_k / kTK);
+    });
     add(kConversion, 0, [&](const GroupItem& it) { return uint64_t(it.conv_rows / kTile) * 2 * ntiles(it); });
     for (uint32_t R = 2; R <= 64; R *= 2)
         add(kForward, R, [&](const GroupItem& it) {
