@@ -142,38 +142,45 @@ pm_status pm_multiply_device_batch(pm_plan* const* plans, const uint32_t* const*
             throw std::invalid_argument("a plan appears twice in one batch (its working buffers hold one product)");
         hip_check(hipSetDevice(device), "hipSetDevice");
         hipStream_t st = stream ? static_cast<hipStream_t>(stream) : plans[0]->lane->stream;
-        std::vector<pm::GroupItem> items;
-        for (size_t i = 0; i < count; ++i) {
-            const pm::Shape& s = plans[i]->host.shape;
-            if (pm::groupable(s))
-                items.push_back(pm::group_item(s, plans[i]->tables, plans[i]->work, a[i], b[i], c[i]));
-            else
-                plans[i]->run(a[i], b[i], c[i], PM_STAGE_OUTPUT, st);  // on its own, ahead of the grouped launches
-        }
-        if (!items.empty()) {
-            // The items and the launches' work lists go into the device's batch
-            // buffer, held (and grown) across calls; a second batch on the same
-            // device waits here until the first has finished with it.
-            BatchBuffer& buf = batch_buffer(device);
-            std::lock_guard<std::mutex> hold(buf.mutex);
-            const size_t need = pm::group_bytes(items.size());
-            if (buf.bytes < need) {
-                if (buf.ptr) hip_check(hipFree(buf.ptr), "hipFree");
-                buf.ptr = nullptr;
-                buf.bytes = 0;
-                hip_check(hipMalloc(&buf.ptr, need), "hipMalloc");
-                buf.bytes = need;
+        // A failure after some launches are queued waits for them before it is
+        // reported: they use the plans' buffers and the batch buffer.
+        try {
+            std::vector<pm::GroupItem> items;
+            for (size_t i = 0; i < count; ++i) {
+                const pm::Shape& s = plans[i]->host.shape;
+                if (pm::groupable(s))
+                    items.push_back(pm::group_item(s, plans[i]->tables, plans[i]->work, a[i], b[i], c[i]));
+                else
+                    plans[i]->run(a[i], b[i], c[i], PM_STAGE_OUTPUT, st);  // on its own, ahead of the grouped launches
             }
-            try {
-                pm::launch_group(items, buf.ptr, st);
-            } catch (...) {
-                (void)hipStreamSynchronize(st);  // nothing queued may outlive the hold on the buffer
-                throw;
+            if (!items.empty()) {
+                // The items and the launches' work lists go into the device's
+                // batch buffer, held (and grown) across calls; a second batch on
+                // the same device waits here until the first has finished with it.
+                BatchBuffer& buf = batch_buffer(device);
+                std::lock_guard<std::mutex> hold(buf.mutex);
+                try {  // the stream is idle again before the buffer is let go
+                    const size_t need = pm::group_bytes(items.size());
+                    if (buf.bytes < need) {
+                        if (buf.ptr) hip_check(hipFree(buf.ptr), "hipFree");
+                        buf.ptr = nullptr;
+                        buf.bytes = 0;
+                        hip_check(hipMalloc(&buf.ptr, need), "hipMalloc");
+                        buf.bytes = need;
+                    }
+                    pm::launch_group(items, buf.ptr, st);
+                    hip_check(hipStreamSynchronize(st), "multiply batch");
+                } catch (...) {
+                    (void)hipStreamSynchronize(st);
+                    throw;
+                }
+                return;
             }
-            hip_check(hipStreamSynchronize(st), "multiply batch");
-        } else {
-            hip_check(hipStreamSynchronize(st), "multiply batch");
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            throw;
         }
+        hip_check(hipStreamSynchronize(st), "multiply batch");
     });
 }
 
