@@ -160,3 +160,35 @@ def test_grouped_batch_matches_device_entry(polymul, workloads, oracle, device_m
                 device_memory.free(ptr)
         for plan in plans:
             plan.close()
+
+
+def test_entry_points_share_a_plan(polymul, workloads, device_memory):
+    """One B-mid plan (N = 32768, L = 868: column-strip chunks on the host
+    path) used through the host-vector entry, the device entry, the grouped
+    batch entry (beside a second plan) and the host entry again, in turn:
+    every result equals the first, so no entry leaves state that another
+    trips over."""
+    print("[gpu-case] entry points share a plan", flush=True)
+    n, width, limbs, q, a, b = workloads.seeded_case("mid")
+    rng = np.random.default_rng(31)
+    q2 = workloads.random_modulus(40, rng)
+    a2, b2 = workloads.uniform_below(q2, 4096, rng), workloads.uniform_below(q2, 4096, rng)
+    with polymul.Plan(n, width, limbs, q) as plan, polymul.Plan(4096, 32 * 40, 40, q2) as other:
+        expect = plan.multiply(a, b)
+        expect2 = other.multiply(a2, b2)
+        bufs = [(device_memory.upload(x), device_memory.upload(y), device_memory.alloc(x.nbytes))
+                for x, y in ((a, b), (a2, b2))]
+        try:
+            for _ in range(2):
+                plan.multiply_device(*bufs[0])
+                assert np.array_equal(device_memory.download(bufs[0][2], a.shape, np.uint32), expect)
+                device_memory.write(bufs[0][2], np.zeros_like(a))
+                polymul.multiply_device_batch([other, plan], [bufs[1][0], bufs[0][0]], [bufs[1][1], bufs[0][1]],
+                                              [bufs[1][2], bufs[0][2]])
+                assert np.array_equal(device_memory.download(bufs[0][2], a.shape, np.uint32), expect)
+                assert np.array_equal(device_memory.download(bufs[1][2], a2.shape, np.uint32), expect2)
+                assert np.array_equal(plan.multiply(a, b), expect)
+        finally:
+            for buffers in bufs:
+                for ptr in buffers:
+                    device_memory.free(ptr)
