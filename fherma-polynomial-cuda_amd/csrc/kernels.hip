@@ -95,17 +95,29 @@ __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b, uint32_t p, uin
     return redc(uint64_t(a) * b, p, pinv);
 }
 
-// REDC of a signed t with |t| < p*2^32; result t*2^-32 mod p in [0, p).
-__device__ __forceinline__ uint32_t redc_signed(int64_t t, uint32_t p, uint32_t pinv) {
-    uint32_t m = uint32_t(uint64_t(t)) * pinv;
-    int64_t r = (t >> 32) - int64_t(__umulhi(m, p)) + int64_t(p);
-    if (r < 0) r += p;
-    if (r >= int64_t(p)) r -= p;
-    return uint32_t(r);
+// 256, 65536 and 2^24 in scalar registers the compiler cannot see through.
+// With literal scales each int64 term becomes a sign extension and a 64-bit
+// shift and add (about four instructions); with a register scale it is one
+// v_mad_i64_i32 (32x32 signed product plus 64-bit addend).
+struct DigitScales {
+    int32_t s8, s16, s24;
+};
+
+__device__ __forceinline__ DigitScales digit_scales() {
+    DigitScales s;
+    asm volatile("s_mov_b32 %0, 0x100" : "=s"(s.s8));
+    asm volatile("s_mov_b32 %0, 0x10000" : "=s"(s.s16));
+    asm volatile("s_mov_b32 %0, 0x1000000" : "=s"(s.s24));
+    return s;
 }
 
-__device__ __forceinline__ int64_t combine4(const v4i& v) {
-    return int64_t(v[0]) + int64_t(v[1]) * 256 + int64_t(v[2]) * 65536 + int64_t(v[3]) * 16777216;
+// base + sum_d 256^d v[d], exact in int64.
+__device__ __forceinline__ int64_t combine4(const v4i& v, const DigitScales& s, int64_t base = 0) {
+    int64_t t = int64_t(v[0]) + base;
+    t += int64_t(v[1]) * int64_t(s.s8);
+    t += int64_t(v[2]) * int64_t(s.s16);
+    t += int64_t(v[3]) * int64_t(s.s24);
+    return t;
 }
 
 // LDS operand tile: 256 rows of 64 bytes; the four 16-byte chunks of a row are
@@ -323,18 +335,24 @@ __device__ __forceinline__ void conversion_body(const int8_t* __restrict__ wtile
     gemm_mainloop(a_panel, b_panel, ksteps, stages, acc);
 
     // Lane holds rows 4*(lane/16)+j (j = digit) of each 16-row tile: one prime.
+    // The digit sum is sum_j c_j*(byte_j - 128) with |c_j| <= p/2 over 4L bytes,
+    // so |sum| <= 256*L*p; adding p*2^31 (0 mod p) to off_k makes t non-negative
+    // and below p*2^32, the range of the unsigned REDC, while 256*L + 1 <= 2^31.
+    // Plans allow at most 16383 primes of < 32 bits covering (q-1)^2, so
+    // q < 2^262144 and L <= 8192: 256*L <= 2^21.
+    const DigitScales ds = digit_scales();
     const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tcol * kTile + wn * kWaveCols;
 #pragma unroll
     for (uint32_t mi = 0; mi < kWaveM; ++mi) {
         const uint32_t kp = (m0 + mi * 16) / 4 + (lane >> 4);
         if (kp >= r) continue;
         const PrimeConst pc = consts[kp];
-        const int64_t off = op ? pc.off_b : pc.off_a;
+        const int64_t off = int64_t(op ? pc.off_b : pc.off_a) + (int64_t(pc.p) << 31);
         uint32_t* out = planes + (size_t(op) * r_pad + kp) * n_pad;
 #pragma unroll
         for (uint32_t ni = 0; ni < kWaveN; ++ni) {
             const uint32_t coef = c0 + ni * 16 + (lane & 15);
-            if (coef < n) out[coef] = redc_signed(combine4(acc[mi][ni]) + off, pc.p, pc.pinv);
+            if (coef < n) out[coef] = redc(uint64_t(combine4(acc[mi][ni], ds, off)), pc.p, pc.pinv);
         }
     }
 }
@@ -378,6 +396,7 @@ __device__ __forceinline__ void crt_body(const int8_t* __restrict__ wtiles, cons
         for (uint32_t ni = 0; ni < kWaveN; ++ni) acc[mi][ni] = v4i{0, 0, 0, 0};
     gemm_mainloop(a_panel, b_panel, ksteps, stages, acc);
 
+    const DigitScales ds = digit_scales();
     const uint32_t m0 = tm * kTile + wm * kWaveRows, c0 = tn * kTile + wn * kWaveCols;
     if constexpr (kByCoefficient) {
 #pragma unroll
@@ -387,7 +406,7 @@ __device__ __forceinline__ void crt_body(const int8_t* __restrict__ wtiles, cons
 #pragma unroll
             for (uint32_t mi = 0; mi < kWaveM; ++mi) {
                 const uint32_t word = (m0 + mi * 16) / 4 + (lane >> 4);
-                if (word < crt_words && coef < n) out[word] = combine4(acc[mi][ni]);
+                if (word < crt_words && coef < n) out[word] = combine4(acc[mi][ni], ds);
             }
         }
     } else {
@@ -399,7 +418,7 @@ __device__ __forceinline__ void crt_body(const int8_t* __restrict__ wtiles, cons
 #pragma unroll
             for (uint32_t ni = 0; ni < kWaveN; ++ni) {
                 const uint32_t coef = c0 + ni * 16 + (lane & 15);
-                if (coef < n) out[coef] = combine4(acc[mi][ni]);
+                if (coef < n) out[coef] = combine4(acc[mi][ni], ds);
             }
         }
     }
