@@ -11,6 +11,10 @@
 // which is what bounds these copies.
 //
 // A product hands the pool one job per chunk (both operands' slices at once).
+// post() starts a job and returns, so the caller can queue the previous
+// chunk's transfer and kernels (HIP calls of 10-50 us each) while the workers
+// copy; wait() then takes what is left and returns once every byte is copied.
+// A job is cut into 256 KB pieces that the threads take from a shared counter.
 // While a product is in flight (set_active(true)) an idle worker spins on the
 // job counter for up to 300 us, so a hand-off between closely spaced chunks
 // costs about a microsecond instead of a condition-variable wake-up; after
@@ -70,10 +74,11 @@ public:
     };
 
     explicit CopyPool(unsigned threads) : parts_(threads < 1 ? 1 : threads) {
-        for (unsigned t = 1; t < parts_; ++t) workers_.emplace_back([this, t] { loop(t); });
+        for (unsigned t = 1; t < parts_; ++t) workers_.emplace_back([this] { loop(); });
     }
 
     ~CopyPool() {
+        wait();
         {
             std::lock_guard<std::mutex> lock(mutex_);
             stop_.store(true, std::memory_order_release);
@@ -105,18 +110,31 @@ public:
         copy(&s, 1);
     }
 
-    // Copies every segment; each is split into 4 KB-aligned shares, one per
-    // thread, and the calling thread takes share 0.  Returns when all are done.
-    // Not reentrant: one job at a time.
+    // Copies every segment and returns when all are done.
     void copy(const Segment* segs, size_t count) {
+        post(segs, count);
+        wait();
+    }
+
+    // Starts copying the segments (the array itself may be reused at once) and
+    // returns; jobs under 1 MB, or a pool of one thread, are copied here and
+    // now.  One job at a time: wait() before the next post().
+    void post(const Segment* segs, size_t count) {
+        wait();
         size_t total = 0;
         for (size_t i = 0; i < count; ++i) total += segs[i].bytes;
         if (total < (size_t(1) << 20) || parts_ == 1) {
             for (size_t i = 0; i < count; ++i) std::memcpy(segs[i].dst, segs[i].src, segs[i].bytes);
             return;
         }
-        segs_.assign(segs, segs + count);
+        pieces_.clear();
+        for (size_t i = 0; i < count; ++i)
+            for (size_t off = 0; off < segs[i].bytes; off += kPiece)
+                pieces_.push_back({static_cast<char*>(segs[i].dst) + off, static_cast<const char*>(segs[i].src) + off,
+                                   std::min(kPiece, segs[i].bytes - off)});
+        next_.store(0, std::memory_order_relaxed);
         pending_.store(parts_ - 1, std::memory_order_relaxed);
+        posted_ = true;
         // Publish, then wake sleepers.  A worker registers as a sleeper under
         // the mutex before it re-checks the counter, so either it sees this
         // job or this thread sees it registered (both sequentially consistent).
@@ -125,25 +143,31 @@ public:
             std::lock_guard<std::mutex> lock(mutex_);
             wake_.notify_all();
         }
-        share(0);
+    }
+
+    // Takes pieces of the posted job until none are left, then waits until
+    // every worker has finished its own and left the job (so the next post()
+    // may rewrite the piece list).  Returns at once with no job posted.
+    void wait() {
+        if (!posted_) return;
+        take();
         while (pending_.load(std::memory_order_acquire) != 0) pause();
+        posted_ = false;
     }
 
 private:
     static void pause() { __builtin_ia32_pause(); }
 
-    void share(unsigned t) {
-        for (const Segment& s : segs_) {
-            const size_t pages = (s.bytes + 4095) / 4096;
-            const size_t first = pages * t / parts_ * 4096;
-            const size_t last = std::min(s.bytes, pages * (t + 1) / parts_ * 4096);
-            if (last > first)
-                copy_streaming(static_cast<char*>(s.dst) + first, static_cast<const char*>(s.src) + first,
-                               last - first);
-        }
+    // Copies pieces of the current job until the counter passes the last;
+    // each piece's streaming stores are fenced before the thread moves on.
+    void take() {
+        const size_t count = pieces_.size();
+        for (size_t i = next_.fetch_add(1, std::memory_order_relaxed); i < count;
+             i = next_.fetch_add(1, std::memory_order_relaxed))
+            copy_streaming(pieces_[i].dst, pieces_[i].src, pieces_[i].bytes);
     }
 
-    void loop(unsigned t) {
+    void loop() {
         // the counter's initial value, not a fresh load: a job published before
         // this thread got here must still be taken
         size_t seen = 0;
@@ -168,7 +192,7 @@ private:
             }
             if (stop_.load(std::memory_order_acquire)) return;
             seen = generation_.load(std::memory_order_acquire);
-            share(t);
+            take();
             pending_.fetch_sub(1, std::memory_order_acq_rel);
         }
     }
@@ -182,8 +206,11 @@ private:
     std::atomic<bool> stop_{false};
     std::atomic<unsigned> sleepers_{0};
     std::atomic<size_t> pokes_{0};
-    std::vector<Segment> segs_;
+    static constexpr size_t kPiece = size_t(256) << 10;
+    std::vector<Segment> pieces_;
+    std::atomic<size_t> next_{0};
     std::atomic<unsigned> pending_{0};
+    bool posted_ = false;  // the calling thread's own state
 };
 
 }  // namespace pm

@@ -202,7 +202,9 @@ struct pm_plan {
     // caller's pageable vectors into pinned staging as [a rows | b rows], one H2D
     // carries both on a copy stream, and the compute stream waits on an event
     // before packing and converting that chunk -- the host copy of chunk k+1
-    // overlaps the H2D of chunk k and the conversion of chunk k-1.  When the
+    // overlaps the H2D of chunk k, the conversion of chunk k-1 and this
+    // thread's HIP calls that queue chunk k (posted to the workers first, so
+    // the copy does not wait for those calls).  When the
     // chunks are column strips, a strip's forward column stages follow the
     // chunk that completes it, under the later chunks' transfers; the row stage
     // needs every coefficient.  On the way out, per chunk: the inverse column
@@ -240,36 +242,54 @@ struct pm_plan {
         // buffers the next call would reuse: wait for them before rethrowing.
         try {
             const pm::Chunking chunks = chunking();
+            // Copy jobs: a job is one chunk's rows of operands [op0, op1).
+            // Nothing overlaps the first chunk's host copy, so with two
+            // operands that chunk goes as two jobs: a's rows, whose H2D then
+            // runs while b's rows are copied, then b's rows.
+            struct Job {
+                uint32_t ch, op0, op1;
+            };
+            Job jobs[pm::kMaxChunks + 1];
+            uint32_t njobs = 0;
             for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-                // the chunk's rows of a, then (two operands) its rows of b, at word
-                // offset nops*local*L in both staging and io_a
-                const size_t rows = chunks.rows_of(ch), base = size_t(nops) * chunks.local[ch] * L;
-                const uint32_t runs = chunks.runs(ch);
+                if (ch == 0 && nops == 2) {
+                    jobs[njobs++] = {0, 0, 1};
+                    jobs[njobs++] = {0, 1, 2};
+                } else {
+                    jobs[njobs++] = {ch, 0, nops};
+                }
+            }
+            // A job's rows, at word offset nops*local*L in both staging and
+            // io_a (the chunk's a rows, then its b rows).
+            pm::CopyPool::Segment segs[2 * pm::kMaxRuns];
+            auto post = [&](const Job& jb) {
+                const size_t rows = chunks.rows_of(jb.ch), base = size_t(nops) * chunks.local[jb.ch] * L;
+                const uint32_t runs = chunks.runs(jb.ch);
                 if (runs > pm::kMaxRuns) throw std::logic_error("a chunk of more runs than kMaxRuns");
-                pm::CopyPool::Segment segs[2 * pm::kMaxRuns];
-                for (uint32_t op = 0; op < nops; ++op)
+                size_t k = 0;
+                for (uint32_t op = jb.op0; op < jb.op1; ++op)
                     for (uint32_t g = 0; g < runs; ++g) {
                         size_t global0, local0, len;
-                        chunks.run(ch, g, s.n, global0, local0, len);
-                        segs[op * runs + g] = {lane->pin_in + base + (op * rows + local0 - chunks.local[ch]) * L,
-                                               (op ? b : a) + global0 * L, len * row_bytes};
+                        chunks.run(jb.ch, g, s.n, global0, local0, len);
+                        segs[k++] = {lane->pin_in + base + (op * rows + local0 - chunks.local[jb.ch]) * L,
+                                     (op ? b : a) + global0 * L, len * row_bytes};
                     }
-                if (ch == 0 && nops == 2) {
-                    // Nothing overlaps the first chunk's host copy, so it goes in
-                    // two halves: a's rows, whose H2D then runs while b's rows are
-                    // copied, then b's rows.
-                    for (uint32_t op = 0; op < 2; ++op) {
-                        pool->copy(segs + op * runs, runs);
-                        hip_check(hipMemcpyAsync(io_a + base + op * rows * L, lane->pin_in + base + op * rows * L,
-                                                 rows * row_bytes, hipMemcpyHostToDevice, copy_in),
-                                  "H2D inputs");
-                    }
-                } else {
-                    pool->copy(segs, size_t(nops) * runs);
-                    hip_check(hipMemcpyAsync(io_a + base, lane->pin_in + base, nops * rows * row_bytes,
-                                             hipMemcpyHostToDevice, copy_in),
-                              "H2D inputs");
-                }
+                pool->post(segs, k);
+            };
+            post(jobs[0]);
+            for (uint32_t j = 0; j < njobs; ++j) {
+                const Job jb = jobs[j];
+                const uint32_t ch = jb.ch;
+                const size_t rows = chunks.rows_of(ch), base = size_t(nops) * chunks.local[ch] * L;
+                pool->wait();
+                const size_t at = base + jb.op0 * rows * L;
+                hip_check(hipMemcpyAsync(io_a + at, lane->pin_in + at, (jb.op1 - jb.op0) * rows * row_bytes,
+                                         hipMemcpyHostToDevice, copy_in),
+                          "H2D inputs");
+                // the workers copy the next job while this thread queues the
+                // chunk's kernels
+                if (j + 1 < njobs) post(jobs[j + 1]);
+                if (jb.op1 < nops) continue;  // the chunk's b rows are still to come
                 hip_check(hipEventRecord(lane->in_ready[ch], copy_in), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
                 const uint32_t* xa = io_a + base;  // pack reads the chunk's k-th tile at rows k*256..
@@ -307,6 +327,7 @@ struct pm_plan {
                 hip_check(hipEventRecord(lane->out_ready[ch], copy_out), "hipEventRecord");
             }
         } catch (...) {
+            pool->wait();  // a posted copy still writes into the staging
             lane->synchronize();
             throw;
         }
