@@ -56,7 +56,7 @@ class Lane {
 public:
     explicit Lane(unsigned copy_threads) {
         try {
-            for (hipStream_t* st : {&stream, &copy_in, &copy_in2, &copy_out})
+            for (hipStream_t* st : {&stream, &copy_in, &copy_out})
                 hip_check(hipStreamCreateWithFlags(st, hipStreamNonBlocking), "hipStreamCreate");
             for (int i = 0; i < kMaxChunks; ++i)
                 for (hipEvent_t* e : {&in_ready[i], &out_ready[i], &reconstructed[i]})
@@ -80,7 +80,7 @@ public:
     }
 
     void synchronize() {
-        for (hipStream_t st : {stream, copy_in, copy_in2, copy_out})
+        for (hipStream_t st : {stream, copy_in, copy_out})
             if (st) (void)hipStreamSynchronize(st);
     }
 
@@ -102,7 +102,7 @@ public:
         pin_words = std::min(pin_in_bytes / (2 * sizeof(uint32_t)), pin_out_bytes / sizeof(uint32_t));
     }
 
-    hipStream_t stream = nullptr, copy_in = nullptr, copy_in2 = nullptr, copy_out = nullptr;
+    hipStream_t stream = nullptr, copy_in = nullptr, copy_out = nullptr;
     hipEvent_t in_ready[kMaxChunks] = {}, out_ready[kMaxChunks] = {}, reconstructed[kMaxChunks] = {};
     hipEvent_t marks[kEvents] = {};
     std::unique_ptr<CopyPool> pool;
@@ -128,9 +128,9 @@ private:
                 if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : marks)
             if (e) (void)hipEventDestroy(e);
-        for (hipStream_t st : {stream, copy_in, copy_in2, copy_out})
+        for (hipStream_t st : {stream, copy_in, copy_out})
             if (st) (void)hipStreamDestroy(st);
-        stream = copy_in = copy_in2 = copy_out = nullptr;
+        stream = copy_in = copy_out = nullptr;
     }
 };
 
@@ -283,7 +283,12 @@ struct pm_plan {
                 const size_t rows = chunks.rows_of(ch), base = size_t(nops) * chunks.local[ch] * L;
                 pool->wait();
                 const size_t at = base + jb.op0 * rows * L;
-                hipStream_t in = (ch & 1) ? lane->copy_in2 : copy_in;
+                // Odd chunks go on the output copy stream, whose transfers all come
+                // after the last input: consecutive chunks' transfers then do not
+                // queue behind each other's completion.  (A fourth stream per lane
+                // would put every lane's compute stream on one of the runtime's
+                // round-robin hardware queues, serializing concurrent plans.)
+                hipStream_t in = (ch & 1) ? copy_out : copy_in;
                 hip_check(hipMemcpyAsync(io_a + at, lane->pin_in + at, (jb.op1 - jb.op0) * rows * row_bytes,
                                          hipMemcpyHostToDevice, in),
                           "H2D inputs");
