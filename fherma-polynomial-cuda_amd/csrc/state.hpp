@@ -242,25 +242,34 @@ struct pm_plan {
         // buffers the next call would reuse: wait for them before rethrowing.
         try {
             const pm::Chunking chunks = chunking();
-            // Copy jobs: a job is one chunk's rows of operands [op0, op1).
-            // Nothing overlaps the first chunk's host copy, so with two
-            // operands that chunk goes as two jobs: a's rows, whose H2D then
-            // runs while b's rows are copied, then b's rows.
+            // Copy jobs: a job is one chunk's rows [r0, r1) of operands
+            // [op0, op1).  Nothing overlaps the first job's host copy, so the
+            // first chunk goes in pieces: with one run (tile ranges) its a rows
+            // in two halves, then (two operands) its b rows, each with its own
+            // H2D, which then runs while the next piece is copied; with
+            // several runs (strips) a's rows, then b's rows.
             struct Job {
                 uint32_t ch, op0, op1;
+                size_t r0, r1;
             };
-            Job jobs[pm::kMaxChunks + 1];
+            Job jobs[pm::kMaxChunks + 2];
             uint32_t njobs = 0;
             for (uint32_t ch = 0; ch < chunks.count; ++ch) {
-                if (ch == 0 && nops == 2) {
-                    jobs[njobs++] = {0, 0, 1};
-                    jobs[njobs++] = {0, 1, 2};
+                const size_t rows = chunks.rows_of(ch);
+                if (ch == 0 && chunks.runs(0) == 1 && rows >= 512) {
+                    jobs[njobs++] = {0, 0, 1, 0, rows / 2};
+                    jobs[njobs++] = {0, 0, 1, rows / 2, rows};
+                    if (nops == 2) jobs[njobs++] = {0, 1, 2, 0, rows};
+                } else if (ch == 0 && nops == 2) {
+                    jobs[njobs++] = {0, 0, 1, 0, rows};
+                    jobs[njobs++] = {0, 1, 2, 0, rows};
                 } else {
-                    jobs[njobs++] = {ch, 0, nops};
+                    jobs[njobs++] = {ch, 0, nops, 0, rows};
                 }
             }
             // A job's rows, at word offset nops*local*L in both staging and
-            // io_a (the chunk's a rows, then its b rows).
+            // io_a (the chunk's a rows, then its b rows).  A job of part of
+            // the rows (one run, one operand) is one segment.
             pm::CopyPool::Segment segs[2 * pm::kMaxRuns];
             auto post = [&](const Job& jb) {
                 const size_t rows = chunks.rows_of(jb.ch), base = size_t(nops) * chunks.local[jb.ch] * L;
@@ -271,6 +280,9 @@ struct pm_plan {
                     for (uint32_t g = 0; g < runs; ++g) {
                         size_t global0, local0, len;
                         chunks.run(jb.ch, g, s.n, global0, local0, len);
+                        if (jb.r1 - jb.r0 < rows) {  // one run: its rows [r0, r1)
+                            global0 += jb.r0, local0 += jb.r0, len = jb.r1 - jb.r0;
+                        }
                         segs[k++] = {lane->pin_in + base + (op * rows + local0 - chunks.local[jb.ch]) * L,
                                      (op ? b : a) + global0 * L, len * row_bytes};
                     }
@@ -282,20 +294,21 @@ struct pm_plan {
                 const uint32_t ch = jb.ch;
                 const size_t rows = chunks.rows_of(ch), base = size_t(nops) * chunks.local[ch] * L;
                 pool->wait();
-                const size_t at = base + jb.op0 * rows * L;
+                const size_t at = base + (jb.op0 * rows + jb.r0) * L;
+                const size_t bytes = jb.r1 - jb.r0 < rows ? (jb.r1 - jb.r0) * row_bytes
+                                                          : (jb.op1 - jb.op0) * rows * row_bytes;
                 // Odd chunks go on the output copy stream, whose transfers all come
                 // after the last input: consecutive chunks' transfers then do not
                 // queue behind each other's completion.  (A fourth stream per lane
                 // would put every lane's compute stream on one of the runtime's
                 // round-robin hardware queues, serializing concurrent plans.)
                 hipStream_t in = (ch & 1) ? copy_out : copy_in;
-                hip_check(hipMemcpyAsync(io_a + at, lane->pin_in + at, (jb.op1 - jb.op0) * rows * row_bytes,
-                                         hipMemcpyHostToDevice, in),
+                hip_check(hipMemcpyAsync(io_a + at, lane->pin_in + at, bytes, hipMemcpyHostToDevice, in),
                           "H2D inputs");
                 // the workers copy the next job while this thread queues the
                 // chunk's kernels
                 if (j + 1 < njobs) post(jobs[j + 1]);
-                if (jb.op1 < nops) continue;  // the chunk's b rows are still to come
+                if (jb.op1 < nops || jb.r1 < rows) continue;  // more of the chunk is still to come
                 hip_check(hipEventRecord(lane->in_ready[ch], in), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(stream, lane->in_ready[ch], 0), "hipStreamWaitEvent");
                 const uint32_t* xa = io_a + base;  // pack reads the chunk's k-th tile at rows k*256..
