@@ -288,6 +288,16 @@ struct pm_plan {
                     }
                 pool->post(segs, k);
             };
+            // Two copy streams remove the hand-off between consecutive chunks'
+            // transfers, but run them two at a time, so each chunk, and its
+            // conversion, lands later.  That pays while a chunk's conversion is
+            // short against its transfer: conversion at about 2.6e15 INT8 ops/s
+            // of 4 * conv_rows * conv_k ops per coefficient, transfer at about
+            // 57 GB/s of 8 * L bytes; under half means conv_rows * conv_k <
+            // 45600 * L.  (N = 32768 with 868 limbs, ratio 0.33: 7.06 against
+            // 7.27 ms per product with one stream; N = 65536 with 1747 limbs,
+            // ratio 0.64: 28.2 against 27.5 ms.)
+            const bool two_streams = uint64_t(s.conv_rows) * s.conv_k < uint64_t(45600) * L;
             post(jobs[0]);
             for (uint32_t j = 0; j < njobs; ++j) {
                 const Job jb = jobs[j];
@@ -297,12 +307,13 @@ struct pm_plan {
                 const size_t at = base + (jb.op0 * rows + jb.r0) * L;
                 const size_t bytes = jb.r1 - jb.r0 < rows ? (jb.r1 - jb.r0) * row_bytes
                                                           : (jb.op1 - jb.op0) * rows * row_bytes;
-                // Odd chunks go on the output copy stream, whose transfers all come
-                // after the last input: consecutive chunks' transfers then do not
-                // queue behind each other's completion.  (A fourth stream per lane
-                // would put every lane's compute stream on one of the runtime's
-                // round-robin hardware queues, serializing concurrent plans.)
-                hipStream_t in = (ch & 1) ? copy_out : copy_in;
+                // With two_streams, odd chunks go on the output copy stream, whose
+                // transfers all come after the last input, so that consecutive
+                // chunks' transfers do not queue behind each other's completion.
+                // (A fourth stream per lane would put every lane's compute stream
+                // on one of the runtime's round-robin hardware queues, serializing
+                // concurrent plans.)
+                hipStream_t in = (two_streams && (ch & 1)) ? copy_out : copy_in;
                 hip_check(hipMemcpyAsync(io_a + at, lane->pin_in + at, bytes, hipMemcpyHostToDevice, in),
                           "H2D inputs");
                 // the workers copy the next job while this thread queues the
