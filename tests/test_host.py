@@ -21,6 +21,25 @@ def test_copy_pool_under_stress(tmp_path):
     assert proc.stdout.startswith("ok ")
 
 
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++ to build the pool check")
+def test_copy_pool_under_thread_sanitizer(tmp_path):
+    """The same stress run, shorter, built with ThreadSanitizer: posted jobs,
+    the piece counter, sleeping and spinning workers and the destructor's
+    drain show no data race."""
+    root = Path(__file__).resolve().parent.parent
+    csrc = root / "fherma-polynomial-cuda_amd" / "csrc"
+    exe = tmp_path / "pool_check_tsan"
+    build = subprocess.run(["g++", "-O1", "-g", "-fsanitize=thread", "-std=c++17", "-I", str(csrc),
+                            str(root / "tests" / "pool_check.cpp"), "-o", str(exe), "-lpthread"],
+                           capture_output=True, text=True, timeout=300)
+    if build.returncode != 0:
+        pytest.skip("no ThreadSanitizer runtime: " + build.stderr[-300:])
+    proc = subprocess.run([str(exe), "120"], capture_output=True, text=True, timeout=600)
+    assert "ThreadSanitizer" not in proc.stderr, proc.stderr[-4000:]
+    assert proc.returncode == 0, proc.stdout[-2000:] + proc.stderr[-2000:]
+    assert proc.stdout.startswith("ok ")
+
+
 @pytest.mark.skipif(shutil.which("g++") is None or not Path("/opt/rocm/include/hip/hip_runtime.h").is_file(),
                     reason="needs g++ and the HIP headers to build the arena check")
 def test_device_arena_with_a_fake_device(tmp_path):
