@@ -239,14 +239,27 @@ class Plan:
         return s
 
     # ---- compute --------------------------------------------------------
-    def multiply(self, a, b):
-        """Host [N][L] uint32 arrays in, host array out (includes transfers)."""
+    def _out(self, out):
+        """`out`, checked to be a writeable contiguous uint32 [N][L] array, or a
+        new one.  A caller running many products can pass the same array each
+        time: a fresh one costs its page faults on the library's first write,
+        about as long as a product at N = 65536 with 55 limbs."""
+        if out is None:
+            return np.empty((self.n, self.limbs), dtype=np.uint32)
+        if (not isinstance(out, np.ndarray) or out.shape != (self.n, self.limbs) or out.dtype != np.uint32
+                or not out.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError("out must be a writeable contiguous uint32 [N][L] array")
+        return out
+
+    def multiply(self, a, b, out=None):
+        """Host [N][L] uint32 arrays in, host array out (includes transfers);
+        c is written into `out` when given."""
         a, _ = _u32(a)
         b, _ = _u32(b)
         expect = (self.n, self.limbs)
         if a.shape != expect or b.shape != expect:
             raise ValueError("operands must be [N][L] = %s" % (expect,))
-        c = np.empty(expect, dtype=np.uint32)
+        c = self._out(out)
         _check(self.lib, self.lib.pm_multiply(self.handle, a.ctypes.data_as(ctypes.c_void_p),
                                               b.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p)))
         return c
@@ -263,9 +276,7 @@ class Plan:
 
     def multiply_finish(self, out=None):
         """Second half: waits and returns c (written into `out` if given)."""
-        c = out if out is not None else np.empty((self.n, self.limbs), dtype=np.uint32)
-        if c.shape != (self.n, self.limbs) or c.dtype != np.uint32 or not c.flags.c_contiguous:
-            raise ValueError("out must be a contiguous uint32 [N][L] array")
+        c = self._out(out)
         _check(self.lib, self.lib.pm_multiply_finish(self.handle, c.ctypes.data_as(ctypes.c_void_p)))
         return c
 
@@ -306,11 +317,12 @@ class Plan:
                                                     ctypes.byref(handle)))
         return ResidentOperand(self, handle)
 
-    def multiply_operand(self, a, operand):
+    def multiply_operand(self, a, operand, out=None):
+        """a times a resident operand; c is written into `out` when given."""
         a, _ = _u32(a)
         if a.shape != (self.n, self.limbs):
             raise ValueError("operand must be [N][L]")
-        c = np.empty((self.n, self.limbs), dtype=np.uint32)
+        c = self._out(out)
         _check(self.lib, self.lib.pm_multiply_operand(self.handle, a.ctypes.data_as(ctypes.c_void_p), operand.handle,
                                                       c.ctypes.data_as(ctypes.c_void_p)))
         return c

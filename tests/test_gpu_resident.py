@@ -38,6 +38,32 @@ def test_resident_operand_device_entry(polymul, workloads, device_memory):
             assert np.array_equal(got, expect)
 
 
+def test_output_array_reused(polymul, workloads):
+    """multiply and multiply_operand write c into a caller's array when given
+    one: the same array twice in a row (filled with junk in between) gives the
+    freshly allocated result, and a wrong shape, dtype or a read-only array is
+    refused before anything runs."""
+    print("[gpu-case] out= arrays", flush=True)
+    n, width, limbs, q, a, b = workloads.seeded_case("small")
+    with polymul.Plan(n, width, limbs, q) as plan:
+        expect = plan.multiply(a, b)
+        out = np.full((n, limbs), 0xDEADBEEF, dtype=np.uint32)
+        for _ in range(2):
+            assert plan.multiply(a, b, out=out) is out
+            assert np.array_equal(out, expect)
+            out.fill(0x5A5A5A5A)
+        with plan.operand(b) as fixed:
+            assert plan.multiply_operand(a, fixed, out=out) is out
+            assert np.array_equal(out, expect)
+        bad = [np.empty((n, limbs + 1), dtype=np.uint32), np.empty((n, limbs), dtype=np.int64),
+               np.empty((limbs, n), dtype=np.uint32).T]
+        ro = np.empty((n, limbs), dtype=np.uint32)
+        ro.flags.writeable = False
+        for x in bad + [ro]:
+            with pytest.raises(ValueError):
+                plan.multiply(a, b, out=x)
+
+
 def test_operand_of_another_plan_is_rejected(polymul, workloads):
     n, width, limbs, q, a, b = workloads.seeded_case("small")
     with polymul.Plan(n, width, limbs, q) as p1, polymul.Plan(n, width, limbs, q) as p2:
