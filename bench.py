@@ -588,6 +588,49 @@ def summarize_kernels(stats_csv, pmc_csv, dest, trace_dir, pmc_dir, clock_csv=No
     Path(dest).write_text("\n".join(lines) + "\n")
 
 
+def summarize_grouped(trace_csv, dest):
+    """Markdown summary of a --kernel-trace run of `bench.py --small-batch-only`:
+    the 64-product small mix run one product at a time (plain, then from four
+    threads: 4 passes each, the first untimed) and as one
+    pm_multiply_device_batch call (4 passes).  The grouped kernels are the ones
+    named *_group*; the others belong to the per-product passes."""
+    import csv
+
+    rows = list(csv.DictReader(open(trace_csv)))
+    disp = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short_kernel(r["Kernel_Name"])) for r in rows]
+    # plan building (table_*, fill_zero) and torch's own kernels are neither form
+    disp = [d for d in disp if not any(x in d[2] for x in ("copyBuffer", "elementwise", "fill", "table_"))]
+    grouped = sorted(d for d in disp if "_group" in d[2])
+    single = [d for d in disp if "_group" not in d[2]]
+    passes_grouped, passes_single = 4, 8
+    per = len(grouped) // passes_grouped
+    spans = []
+    for k in range(passes_grouped):
+        part = grouped[k * per:(k + 1) * per]
+        spans.append((part[-1][1] - part[0][0]) / 1e6)
+    busy_g = sum(e - s for s, e, _ in grouped) / 1e6 / passes_grouped
+    busy_s = sum(e - s for s, e, _ in single) / 1e6 / passes_single
+    lines = ["# Small-product mix: one launch per product and stage against one per stage", "",
+             "Command (from the repo root, on one MI355X):", "", "```",
+             f"rocprofv3 --kernel-trace --output-format csv -d {trace_csv.parent} -o small -- "
+             "python3 bench.py --small-batch-only", "```", "",
+             "The 64 products of `workloads.batch_small_products()` (N = 2^10..2^13, L log-uniform in [8, 128], "
+             "each with its own q), operands in HBM. `pm_multiply_device` one product at a time and from four "
+             "threads (4 passes each, the first untimed), then `pm_multiply_device_batch` (4 passes). Summarised by "
+             "`python3 bench.py --summarize-grouped <kernel trace csv>`.", "",
+             "| form | kernel dispatches per pass | kernel time per pass (ms, summed) |", "|---|---|---|",
+             "| one product at a time (plain and four threads) | %d | %.3f |" % (len(single) // passes_single, busy_s),
+             "| pm_multiply_device_batch | %d | %.3f |" % (per, busy_g), "",
+             "Grouped passes, first kernel start to last kernel end (ms): " + ", ".join("%.3f" % x for x in spans),
+             "", "| grouped kernel | dispatches per pass | avg ms |", "|---|---|---|"]
+    by = {}
+    for s_, e, name in grouped:
+        by.setdefault(name, []).append((e - s_) / 1e6)
+    for name, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        lines.append("| %s | %d | %.4f |" % (name, len(v) // passes_grouped, sum(v) / len(v)))
+    Path(dest).write_text("\n".join(lines) + "\n")
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -606,6 +649,11 @@ def main():
                         help="also run one plain 1 GiB device-to-device copy (counter calibration)")
     parser.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                         help="write profiles/traffic_large.json from two counter runs, then exit (no GPU)")
+    parser.add_argument("--small-batch-only", action="store_true",
+                        help="run only the device-resident small-product mix (plain, threads, grouped) and exit")
+    parser.add_argument("--summarize-grouped", metavar="TRACE_CSV",
+                        help="write profiles/grouped_small.md from a --kernel-trace of --small-batch-only, then "
+                             "exit (no GPU)")
     parser.add_argument("--summarize-kernels", nargs="+", metavar="CSV",
                         help="write profiles/kernels_large.md from a kernel-stats csv and optionally an "
                              "SQ counter csv, a GRBM_GUI_ACTIVE counter csv and a VALU counter csv, then exit "
@@ -616,6 +664,9 @@ def main():
         return
     if args.summarize_traffic:
         summarize_traffic(*args.summarize_traffic, ROOT / "profiles" / "traffic_large.json")
+        return
+    if args.summarize_grouped:
+        summarize_grouped(Path(args.summarize_grouped), ROOT / "profiles" / "grouped_small.md")
         return
     if args.summarize_kernels:
         stats = Path(args.summarize_kernels[0])
@@ -659,6 +710,12 @@ def main():
         del src, dst
 
     workloads = _load("workloads")
+    if args.small_batch_only:
+        # the small-product mix alone (for a kernel trace of its three forms)
+        leg = device_batch_leg(polymul, torch, workloads.batch_small_products(), args.lanes,
+                               golden_key="batch_small")
+        print(json.dumps({"batched_device_small": leg}))
+        return
     t_gen = time.perf_counter()
     n, width, limbs, q, a, b = workloads.seeded_case("large")
     gen_s = time.perf_counter() - t_gen
