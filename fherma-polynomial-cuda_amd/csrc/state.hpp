@@ -256,7 +256,9 @@ struct pm_plan {
             uint32_t njobs = 0;
             for (uint32_t ch = 0; ch < chunks.count; ++ch) {
                 const size_t rows = chunks.rows_of(ch);
-                if (ch == 0 && chunks.runs(0) == 1 && rows >= 512) {
+                // (halves only when a's rows are 2 MB or more: below that the
+                // extra transfer's hand-off costs more than the copy it hides)
+                if (ch == 0 && chunks.runs(0) == 1 && rows * row_bytes >= (size_t(2) << 20)) {
                     jobs[njobs++] = {0, 0, 1, 0, rows / 2};
                     jobs[njobs++] = {0, 0, 1, rows / 2, rows};
                     if (nops == 2) jobs[njobs++] = {0, 1, 2, 0, rows};
