@@ -60,9 +60,24 @@ def test_kernel_table_lists_every_kernel(tmp_path):
     dest = tmp_path / "kernels.md"
     bench.summarize_kernels(stats, None, dest, "out/trace", None)
     text = dest.read_text()
-    assert "| crt_gemm | 6 | 5.1000 | 5.0000 | 30.600 | 60.00 |" in text
-    assert "| reconstruct | 6 | 0.7800 | 0.7600 | 4.680 | 9.20 |" in text
+    # no kernel trace beside the stats: no timed average
+    assert "| crt_gemm | 6 | 5.1000 | - | 5.0000 | 30.600 | 60.00 |" in text
+    assert "| reconstruct | 6 | 0.7800 | - | 0.7600 | 4.680 | 9.20 |" in text
     assert "-d out/trace -o large" in text
+    # with the trace: the first (warm-up) launch is dropped, whatever order the rows come in
+    trace = stats.parent / "large_kernel_trace.csv"
+    with open(trace, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        w.writeheader()
+        starts = [0, 100, 200, 300, 400, 500]
+        durations = [9.0, 5.0, 5.2, 4.8, 5.0, 5.0]  # ms; the warm-up launch is the slow one
+        for t, d in reversed(list(zip(starts, durations))):
+            w.writerow({"Kernel_Name": "pm::(anonymous namespace)::crt_gemm(long*)",
+                        "Start_Timestamp": t * 10 ** 6, "End_Timestamp": t * 10 ** 6 + int(d * 1e6)})
+    bench.summarize_kernels(stats, None, dest, "out/trace", None)
+    text = dest.read_text()
+    assert "| crt_gemm | 6 | 5.1000 | 5.0000 | 5.0000 | 30.600 | 60.00 |" in text
+    assert "| reconstruct | 6 | 0.7800 | - | 0.7600 | 4.680 | 9.20 |" in text
 
 
 def test_clock_is_busy_cycles_per_xcd_over_the_dispatch(tmp_path):
