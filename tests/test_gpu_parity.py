@@ -254,6 +254,30 @@ def test_wide_q_reconstruction_edges(polymul, workloads, oracle, n, limbs, kind)
         _check_product(plan, m, m, q, oracle)
 
 
+# q needs no structure: an even q and a power of two (q = 2^64 in three limbs,
+# so Barrett's window and the final subtraction see a q with zero low words)
+# give the oracle's product, also for the all-(q-1) operands whose results sit
+# next to a multiple of q; 70 limbs take the wave-per-coefficient reconstruction.
+@pytest.mark.parametrize("n,limbs,kind", [(64, 3, "power"), (1024, 8, "even"), (256, 70, "even")])
+def test_even_modulus(polymul, workloads, oracle, n, limbs, kind):
+    _announce("even q", n, limbs, kind)
+    rng = np.random.default_rng(7700 + n + limbs)
+    if kind == "power":
+        q = np.zeros(limbs, dtype=np.uint32)
+        q[-1] = 1
+    else:
+        q = workloads.random_modulus(limbs, rng)
+        q[0] &= np.uint32(0xFFFFFFFE)
+    assert oracle.words_to_int(q) % 2 == 0
+    a = workloads.uniform_below(q, n, rng)
+    b = workloads.uniform_below(q, n, rng)
+    with polymul.Plan(n, 32 * limbs, limbs, q) as plan:
+        _check_product(plan, a, b, q, oracle)
+        m = _edge_operands(workloads, q, n, "max")
+        _check_product(plan, m, m, q, oracle)
+        _check_product(plan, m, b, q, oracle)
+
+
 # Host products large enough for one chunk per column strip move their
 # coefficients in strips (256 columns of the [rows][2048] matrix the transforms
 # use): 2 rows (one chunk per strip), 4 and 8 rows (the first and last strips
