@@ -351,12 +351,19 @@ struct pm_plan {
                     hip_check(hipEventRecord(lane->out_ready[ch], stream), "hipEventRecord");
                     continue;
                 }
+                // With two_streams the D2Hs alternate too (the input stream is idle
+                // by now), so that a chunk's D2H does not wait behind the previous
+                // one's completion event.  Where a chunk's conversion is long
+                // against its transfer the chunks are large, and then the host's
+                // copy of each into the caller's vector, whose fresh pages fault on
+                // first write, is what sets the pace: D2Hs landing in pairs only
+                // build a backlog there, so they stay on one stream.
+                hipStream_t out = (two_streams && (ch & 1)) ? copy_in : copy_out;
                 hip_check(hipEventRecord(lane->reconstructed[ch], stream), "hipEventRecord");
-                hip_check(hipStreamWaitEvent(copy_out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
-                hip_check(hipMemcpyAsync(lane->pin_out + at, io_c + at, rows * row_bytes, hipMemcpyDeviceToHost,
-                                         copy_out),
+                hip_check(hipStreamWaitEvent(out, lane->reconstructed[ch], 0), "hipStreamWaitEvent");
+                hip_check(hipMemcpyAsync(lane->pin_out + at, io_c + at, rows * row_bytes, hipMemcpyDeviceToHost, out),
                           "D2H c");
-                hip_check(hipEventRecord(lane->out_ready[ch], copy_out), "hipEventRecord");
+                hip_check(hipEventRecord(lane->out_ready[ch], out), "hipEventRecord");
             }
         } catch (...) {
             pool->wait();  // a posted copy still writes into the staging
